@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""BT-MZ timestep throughput on one MI355X.
+
+Workload: BT-MZ class B, 304x208x17 grid in 8x8 uneven zones, FP64, dt=0.0003
+(the configuration the headline metric is quoted on).  A "step" is one
+timestep over all zones (exch_qbc + compute_rhs + x/y/z block-tridiagonal
+solves + add).  The timed region is --steps timesteps of the batched,
+hipGraph-replayed form, with every field already resident in HBM; the same
+count of per-zone-launch steps is timed for the batched/per-zone ratio, and a
+batched pass with HIP events around each phase gives the dominant kernel's
+duration for the roofline.  One JSON line is printed.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Algorithmic bytes per grid point and launch: 5-vector fields (40 B per
+# point) each read or written once: rhs reads u and forcing and writes rhs;
+# each solve reads u and rhs and writes rhs; the z solve with add folded in
+# reads u and rhs and writes u.
+FIELDS_PER_PHASE = {"exch": None, "rhs": 3, "x_solve": 3, "y_solve": 3, "z_solve+add": 3}
+PHASES = ["exch", "rhs", "x_solve", "y_solve", "z_solve+add"]
+
+
+def face_points(zones):
+    n = 0
+    for z in range(zones.count):
+        n += 2 * (zones.ny[z] - 2) * (zones.nz[z] - 2) + 2 * (zones.nx[z] - 2) * (zones.nz[z] - 2)
+    return n
+
+
+def cpu_baseline(params, budget_s, threads):
+    """The CPU oracle (C++ restatement of the reference's Julia BT path) timed
+    on this host's cores over a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleBT
+    import npbapps_amd as P
+    o = OracleBT.for_class(params, threads=threads)
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter() - t0
+    n = max(1, int(budget_s / max(t1, 1e-3)) - 1)
+    t0 = time.perf_counter()
+    o.step(n)
+    t = time.perf_counter() - t0
+    o.close()
+    zones, _ = P.zones_for("BT", "B")
+    mflop = P.mflops_per_step("BT", zones)
+    return {"value": round(mflop * n / t, 2), "unit": "Mop/s", "cores": threads,
+            "kind": "port",
+            "sample": f"BT-MZ class B, {n} timesteps after 1 warm-up step, "
+                      f"{threads} OpenMP threads over zones, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-perzone", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
+              "multi-zone path would shard by zone across GPUs, which is not built here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import npbapps_amd as P
+    zones, p = P.zones_for("BT", "B")
+    K, W = args.steps, args.warmup
+    mflop_step = P.mflops_per_step("BT", zones)
+    npts = zones.npoints
+
+    ctx = P.Context("BT", zones, p.dt)
+    ctx.init_exact()
+    ctx.step(W, "graph")
+    ctx.init_exact()
+    t_graph = ctx.step(K, "graph")
+    xcr, xce, _ = ctx.norms()
+    verified = None
+    if K == p.niter:
+        verified, _ = P.verify.verify("BT", "B", p.dt, K, xcr, xce)
+
+    t_perzone = None
+    if not args.no_perzone:
+        ctx.init_exact()
+        ctx.step(W, "perzone")
+        ctx.init_exact()
+        t_perzone = ctx.step(K, "perzone")
+
+    ctx.init_exact()
+    ctx.step(W, "batched")
+    ctx.init_exact()
+    t_batched = ctx.step(K, "batched")
+    ctx.init_exact()
+    ph = ctx.phase_times(K)[:5]
+    sched = ctx.describe()
+    ctx.close()
+
+    per_launch_s = {name: ph[i] / K for i, name in enumerate(PHASES)}
+    dom = max(PHASES[1:], key=lambda n: per_launch_s[n])
+    alg_bytes = FIELDS_PER_PHASE[dom] * 40 * npts
+    achieved = alg_bytes / per_launch_s[dom] / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "bt_class_b_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dom)
+
+    value = mflop_step * K / t_graph
+    out = {
+        "metric": "Mop/s & grid-pt-steps/s FP64 1 GPU: batched vs per-zone; adjoint/forward ratio",
+        "value": round(value, 2),
+        "unit": "Mop/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(t_graph / K * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: the reference's exact-solution initial field and forcing "
+                "(deterministic, no dataset)",
+        "config": {"workload": "BT-MZ class B: 304x208x17 grid, 8x8 uneven zones "
+                               "(x 16..72, y 11..49), 200 timesteps at dt=0.0003",
+                   "mode": "batched + hipGraph", "fp": "fma", "schedule": sched},
+        "grid_pt_steps_per_s": round(npts * K / t_graph, 1),
+        "verified": verified,
+        "batched_mops": round(mflop_step * K / t_batched, 2),
+        "perzone": None if t_perzone is None else {
+            "value": round(mflop_step * K / t_perzone, 2), "unit": "Mop/s",
+            "ms_per_step": round(t_perzone / K * 1e3, 5)},
+        "graph_vs_perzone": None if t_perzone is None else round(t_perzone / t_graph, 3),
+        "adjoint_forward_ratio": None,
+        "phase_ms_per_step": {n: round(per_launch_s[n] * 1e3, 5) for n in PHASES},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": alg_bytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds, threads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

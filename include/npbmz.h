@@ -1,0 +1,158 @@
+/* npbmz.h - C ABI of libnpbmz, the MI355X (gfx950) multi-zone BT/SP/LU timestep.
+ *
+ * The reference (NPBApps.jl) has no FFI: its node loop calls exch_qbc and a
+ * threaded per-zone adi/ssor directly in Julia.  These entry points replace
+ * exactly that loop body and the node-side verify numerics, so that the Julia
+ * driver (go, zone_setup, initialize/exact_rhs, class verification) can stay
+ * as it is and reach the GPU through ccall.  Citations below are to the
+ * reference's files.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative NPBMZ_E* code on
+ *     failure; nothing aborts or throws across the ABI.  npbmz_last_error()
+ *     gives a message for the last failure on a context.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - A context owns device memory, one HIP stream and captured graphs; it is
+ *     not thread safe (one caller thread per context).
+ *   - Zones are numbered from 0 (the reference numbers them from 1).
+ *   - All arithmetic is IEEE binary64.
+ */
+#ifndef NPBMZ_H
+#define NPBMZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* benchmark selector */
+#define NPBMZ_BT 0
+#define NPBMZ_SP 1
+#define NPBMZ_LU 2
+
+/* error codes */
+#define NPBMZ_OK 0
+#define NPBMZ_EINVAL -1   /* bad argument */
+#define NPBMZ_EHIP -2     /* HIP runtime failure (message holds the HIP error) */
+#define NPBMZ_ENOMEM -3   /* device allocation failed */
+#define NPBMZ_ENOTSUP -4  /* feature not built for this benchmark */
+#define NPBMZ_ENODEV -5   /* no usable gfx950 device */
+
+/* creation flags */
+#define NPBMZ_FLAG_STRICT_FP 1 /* kernels built without fused multiply-add:
+                                  bit-identical to the reference's CPU path */
+
+/* fields for put/get */
+#define NPBMZ_FIELD_U 0       /* conserved variables u (BT/SP/LU) */
+#define NPBMZ_FIELD_FORCING 1 /* forcing (BT/SP), frct (LU) */
+#define NPBMZ_FIELD_RHS 2     /* rhs (BT/SP), rsd (LU) */
+
+/* host array axis order for put/get */
+#define NPBMZ_ORDER_MIJK 0 /* Julia [m,i,j,k] (BT u/forcing/rhs, LU u/rsd/frct) */
+#define NPBMZ_ORDER_IJKM 1 /* Julia [i,j,k,m] (SP u/forcing/rhs) */
+
+/* step modes */
+#define NPBMZ_MODE_PERZONE 0 /* one launch per zone per phase */
+#define NPBMZ_MODE_BATCHED 1 /* all zones in one launch per phase */
+#define NPBMZ_MODE_GRAPH 2   /* batched, whole timestep replayed from a hipGraph */
+
+/* phases for npbmz_run_phase (testing and profiling) */
+#define NPBMZ_PHASE_EXCH 0  /* exch_qbc face copy */
+#define NPBMZ_PHASE_RHS 1   /* compute_rhs (BT/SP), rhs (LU) */
+#define NPBMZ_PHASE_XSOLVE 2
+#define NPBMZ_PHASE_YSOLVE 3
+#define NPBMZ_PHASE_ZSOLVE 4
+#define NPBMZ_PHASE_ADD 5
+
+typedef struct npbmz_ctx npbmz_ctx;
+
+/* One multi-zone problem.  nx/ny/nz/iz_* are per zone, in the order that
+ * zone_setup produces them (src/BT/zone_setup.jl:53-74).  gx/gy/gz and
+ * x_zones/y_zones feed set_constants (src/BT/set_constants.jl:53-130). */
+typedef struct {
+  int32_t bench;   /* NPBMZ_BT / NPBMZ_SP / NPBMZ_LU */
+  int32_t nzones;  /* x_zones * y_zones */
+  int32_t x_zones, y_zones;
+  int32_t gx_size, gy_size, gz_size;
+  const int32_t *nx, *ny, *nz;
+  const int32_t *iz_west, *iz_east, *iz_south, *iz_north; /* 0-based */
+  double dt;
+  double omega; /* LU only (src/LU/lu_data.jl:8); ignored otherwise */
+} npbmz_problem;
+
+/* Create a context on the current HIP device: validates the zone table,
+ * computes the stencil constants, allocates every zone's fields in HBM.
+ * Replaces the allocation part of perform (src/BT/bt-node.jl:63-93). */
+int npbmz_create(const npbmz_problem* prob, int32_t flags, npbmz_ctx** out);
+
+/* Release everything the context owns. */
+void npbmz_destroy(npbmz_ctx* ctx);
+
+/* Message for the last failure on ctx (or a creation failure if ctx is NULL). */
+const char* npbmz_last_error(const npbmz_ctx* ctx);
+
+/* Copy one zone's field between a host array in the reference's layout and
+ * HBM.  host points at the parent array of the Julia OffsetArray; lo[d]/hi[d]
+ * are its axis bounds (d = 0..3 in `order`), e.g. BT u is
+ * [1:5, -2:nx+2, -2:ny+2, -2:nz+2] (src/BT/bt_data.jl:143-144).  Only the
+ * zone's grid points are transferred (get leaves the rest untouched).
+ * Replaces the Julia arrays being owned by the node (bt_data.jl:270-326). */
+int npbmz_put_field(npbmz_ctx* ctx, int32_t zone, int32_t field, const double* host,
+                    const int64_t lo[4], const int64_t hi[4], int32_t order);
+int npbmz_get_field(npbmz_ctx* ctx, int32_t zone, int32_t field, double* host,
+                    const int64_t lo[4], const int64_t hi[4], int32_t order);
+
+/* Device-side initial condition: initialize + exact_rhs (BT/SP,
+ * src/BT/initialize.jl:26-225, src/BT/exact_rhs.jl:5-370) or
+ * setbv + setiv + erhs (LU).  Same values as the host routines. */
+int npbmz_init_exact(npbmz_ctx* ctx);
+
+/* Run nsteps timesteps: for each, exch_qbc then every zone's adi (BT/SP) or
+ * ssor (LU).  Replaces the STEP loop body of perform
+ * (src/BT/bt-node.jl:280-417, src/SP/sp-node.jl:252-391,
+ * src/LU/lu-node.jl:338-428).  If elapsed_ms is not NULL it receives the
+ * HIP-event time of the whole call (inputs already resident in HBM). */
+int npbmz_step(npbmz_ctx* ctx, int32_t nsteps, int32_t mode, double* elapsed_ms);
+
+/* Run one phase over all zones (testing and profiling). */
+int npbmz_run_phase(npbmz_ctx* ctx, int32_t phase, int32_t mode);
+
+/* Run nsteps batched timesteps with a HIP event pair around every phase's
+ * launch on the context's stream.  phase_ms[p] (p = NPBMZ_PHASE_*) receives
+ * the summed time of phase p over the steps; the z sweep folds add, so
+ * phase_ms[NPBMZ_PHASE_ADD] is 0 for BT.  Per-phase counterpart of the
+ * reference's itimer timers (src/BT/bt_data.jl:50-76). */
+int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]);
+
+/* Node-side verification numerics on the current state, in the reference's
+ * order: per zone error_norm, compute_rhs without a new exchange, rhs_norm;
+ * xcr = sum over zones of rms_rhs/dt, xce = sum of rms_err
+ * (src/BT/verify.jl:411-501, src/SP/verify.jl:411-515).  For LU, xcr and xce
+ * are the summed l2norm of rsd and ERROR, and xci (may be NULL) the summed
+ * surface integral (src/LU/lu-node.jl:430-464). */
+int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci);
+
+/* Per-zone raw residual/error norms of the last npbmz_norms call
+ * (nzones x 5 each, zone-major); either pointer may be NULL. */
+int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err);
+
+/* Autotune the batched launch schedule (zone order, workgroups per zone,
+ * persistent vs one-shot grid) by timing candidate schedules for about
+ * budget_ms; the choice is kept in the context and used by the batched and
+ * graph modes.  No counterpart in the reference (its zone->thread schedule
+ * is Threads.@threads over zones, src/BT/bt-node.jl:318). */
+int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
+
+/* Describe the schedule currently in use (for logs); writes a NUL-terminated
+ * string of at most cap bytes. */
+int npbmz_describe(const npbmz_ctx* ctx, char* buf, int32_t cap);
+
+/* Number of zones and total grid points held by ctx. */
+int npbmz_size(const npbmz_ctx* ctx, int32_t* nzones, int64_t* npoints);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPBMZ_H */

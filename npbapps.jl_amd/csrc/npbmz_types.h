@@ -1,0 +1,47 @@
+// Types shared by the host side of libnpbmz and its gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace npbmz {
+
+// One zone as the kernels see it.  Fields live in HBM as one array per field
+// for all zones; zone z's grid point (i, j, k), 0 <= i < nx etc., is at point
+// index base + (k*ny + j)*nx + i and its 5 components are contiguous
+// (the reference's [m,i,j,k] order with m fastest).
+struct ZoneDev {
+  int32_t nx, ny, nz, pad;
+  int64_t base;
+};
+
+// A unit of work: `count` consecutive items (points or lines) of zone `zone`
+// starting at item `first`, in the zone's own item numbering.
+struct Task {
+  int32_t zone;
+  int32_t first;
+};
+
+// Stencil constants for BT and SP (src/BT/set_constants.jl:1-130 and
+// src/SP/set_constants.jl), grouped by direction d = 0 (xi), 1 (eta),
+// 2 (zeta) so a kernel templated on the direction indexes them statically.
+struct StencilK {
+  double dt;
+  double c1, c2, c3, c4, c5, c1c2, c1c5, c3c4, c1345, con43, bt, c2iv, con16;
+  double dssp, c4dssp, c5dssp, dtdssp, comz1, comz4, comz5, comz6;
+  double dnm1[3];     // dnxm1, dnym1, dnzm1
+  double t1[3];       // tx1, ty1, tz1
+  double t2[3];       // tx2, ty2, tz2
+  double t3[3];       // tx3, ty3, tz3
+  double dd[3][5];    // dx1..dx5, dy1..dy5, dz1..dz5
+  double ddt1[3][5];  // dx1tx1..dx5tx1, dy1ty1.., dz1tz1..
+  double con1[3];     // xxcon1, yycon1, zzcon1
+  double con2[3];     // xxcon2, ...
+  double con3[3];
+  double con4[3];
+  double con5[3];
+  double dtt1[3];     // dttx1, dtty1, dttz1
+  double dtt2[3];     // dttx2, dtty2, dttz2
+  double c2dtt1[3];   // c2dttx1, ...
+  double dmax[3];     // dxmax, dymax, dzmax
+};
+
+}  // namespace npbmz

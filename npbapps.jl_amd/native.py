@@ -1,0 +1,224 @@
+"""ctypes binding of libnpbmz (the C ABI declared in include/npbmz.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: every entry point raises NpbmzError when the library is missing or
+a call fails, so nothing silently runs anywhere but on the GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnpbmz.so")
+
+BT, SP, LU = 0, 1, 2
+BENCH = {"BT": BT, "SP": SP, "LU": LU}
+
+FLAG_STRICT_FP = 1
+FIELD_U, FIELD_FORCING, FIELD_RHS = 0, 1, 2
+ORDER_MIJK, ORDER_IJKM = 0, 1
+MODE_PERZONE, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+MODES = {"perzone": MODE_PERZONE, "batched": MODE_BATCHED, "graph": MODE_GRAPH}
+PHASE_EXCH, PHASE_RHS, PHASE_XSOLVE, PHASE_YSOLVE, PHASE_ZSOLVE, PHASE_ADD = range(6)
+
+# Every symbol include/npbmz.h declares.
+EXPORTS = (
+    "npbmz_create", "npbmz_destroy", "npbmz_last_error", "npbmz_put_field",
+    "npbmz_get_field", "npbmz_init_exact", "npbmz_step", "npbmz_run_phase", "npbmz_phase_times",
+    "npbmz_norms", "npbmz_zone_norms", "npbmz_tune", "npbmz_describe", "npbmz_size",
+)
+
+
+class NpbmzError(RuntimeError):
+    """A libnpbmz call failed (negative return code) or the library is absent."""
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("bench", ctypes.c_int32), ("nzones", ctypes.c_int32),
+        ("x_zones", ctypes.c_int32), ("y_zones", ctypes.c_int32),
+        ("gx_size", ctypes.c_int32), ("gy_size", ctypes.c_int32), ("gz_size", ctypes.c_int32),
+        ("nx", ctypes.POINTER(ctypes.c_int32)), ("ny", ctypes.POINTER(ctypes.c_int32)),
+        ("nz", ctypes.POINTER(ctypes.c_int32)),
+        ("iz_west", ctypes.POINTER(ctypes.c_int32)), ("iz_east", ctypes.POINTER(ctypes.c_int32)),
+        ("iz_south", ctypes.POINTER(ctypes.c_int32)), ("iz_north", ctypes.POINTER(ctypes.c_int32)),
+        ("dt", ctypes.c_double), ("omega", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libnpbmz and declare its prototypes; raise NpbmzError if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NpbmzError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    dp = ctypes.POINTER(dbl)
+    i64p = ctypes.POINTER(i64)
+    lib.npbmz_create.argtypes = [ctypes.POINTER(Problem), i32, ctypes.POINTER(P)]
+    lib.npbmz_destroy.argtypes = [P]
+    lib.npbmz_destroy.restype = None
+    lib.npbmz_last_error.argtypes = [P]
+    lib.npbmz_last_error.restype = ctypes.c_char_p
+    for f in ("npbmz_put_field", "npbmz_get_field"):
+        getattr(lib, f).argtypes = [P, i32, i32, dp, i64p, i64p, i32]
+    lib.npbmz_init_exact.argtypes = [P]
+    lib.npbmz_step.argtypes = [P, i32, i32, dp]
+    lib.npbmz_run_phase.argtypes = [P, i32, i32]
+    lib.npbmz_phase_times.argtypes = [P, i32, dp]
+    lib.npbmz_norms.argtypes = [P, dp, dp, dp]
+    lib.npbmz_zone_norms.argtypes = [P, dp, dp]
+    lib.npbmz_tune.argtypes = [P, i32]
+    lib.npbmz_describe.argtypes = [P, ctypes.c_char_p, i32]
+    lib.npbmz_size.argtypes = [P, ctypes.POINTER(i32), i64p]
+    for f in EXPORTS:
+        if f not in ("npbmz_destroy", "npbmz_last_error"):
+            getattr(lib, f).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i32(a):
+    return (ctypes.c_int32 * len(a))(*[int(x) for x in a])
+
+
+class Context:
+    """One libnpbmz context: a multi-zone problem resident in HBM."""
+
+    def __init__(self, bench, zones, dt, omega=0.0, strict=False):
+        lib = load()
+        self._lib = lib
+        self.bench = BENCH[bench] if isinstance(bench, str) else int(bench)
+        self.zones = zones
+        self.dt = float(dt)
+        n = zones.count
+        self._keep = [_i32(zones.nx), _i32(zones.ny), _i32(zones.nz), _i32(zones.west),
+                      _i32(zones.east), _i32(zones.south), _i32(zones.north)]
+        p = Problem(self.bench, n, zones.x_zones, zones.y_zones, zones.gx, zones.gy, zones.gz,
+                    *self._keep, float(dt), float(omega))
+        h = ctypes.c_void_p()
+        rc = lib.npbmz_create(ctypes.byref(p), FLAG_STRICT_FP if strict else 0, ctypes.byref(h))
+        if rc != 0:
+            raise NpbmzError(f"npbmz_create -> {rc}: {lib.npbmz_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.npbmz_last_error(self._h).decode()
+            raise NpbmzError(f"{what} -> {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.npbmz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- fields ------------------------------------------------------------
+    @staticmethod
+    def _check_extent(host, lo, hi):
+        n = 1
+        for a, b in zip(lo, hi):
+            n *= int(b) - int(a) + 1
+        if host.size != n:
+            raise NpbmzError(f"host array has {host.size} elements, bounds describe {n}")
+
+    def put_field(self, zone, field, host, lo, hi, order=ORDER_MIJK):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        self._check_extent(host, lo, hi)
+        lo_a = (ctypes.c_int64 * 4)(*lo)
+        hi_a = (ctypes.c_int64 * 4)(*hi)
+        self._check(self._lib.npbmz_put_field(self._h, zone, field, _dptr(host), lo_a, hi_a,
+                                              order), "npbmz_put_field")
+
+    def get_field(self, zone, field, host, lo, hi, order=ORDER_MIJK):
+        assert host.dtype == np.float64 and host.flags["C_CONTIGUOUS"]
+        self._check_extent(host, lo, hi)
+        lo_a = (ctypes.c_int64 * 4)(*lo)
+        hi_a = (ctypes.c_int64 * 4)(*hi)
+        self._check(self._lib.npbmz_get_field(self._h, zone, field, _dptr(host), lo_a, hi_a,
+                                              order), "npbmz_get_field")
+        return host
+
+    def get_dense(self, zone, field):
+        """Zone field as a (nz, ny, nx, 5) array over the zone's grid points
+        (the reference's [m,i,j,k] column-major order)."""
+        nx, ny, nz = self.zones.nx[zone], self.zones.ny[zone], self.zones.nz[zone]
+        org = 1 if self.bench == LU else 0
+        out = np.zeros((nz, ny, nx, 5))
+        return self.get_field(zone, field, out, (1, org, org, org),
+                              (5, nx - 1 + org, ny - 1 + org, nz - 1 + org))
+
+    def put_dense(self, zone, field, arr):
+        nx, ny, nz = self.zones.nx[zone], self.zones.ny[zone], self.zones.nz[zone]
+        org = 1 if self.bench == LU else 0
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(nz, ny, nx, 5)
+        self.put_field(zone, field, arr, (1, org, org, org),
+                       (5, nx - 1 + org, ny - 1 + org, nz - 1 + org))
+
+    # -- compute -------------------------------------------------------------
+    def init_exact(self):
+        self._check(self._lib.npbmz_init_exact(self._h), "npbmz_init_exact")
+
+    def step(self, nsteps, mode="graph"):
+        """Run nsteps timesteps; returns the HIP-event time in seconds."""
+        ms = ctypes.c_double(0.0)
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        self._check(self._lib.npbmz_step(self._h, int(nsteps), m, ctypes.byref(ms)), "npbmz_step")
+        return ms.value * 1e-3
+
+    def run_phase(self, phase, mode="batched"):
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        self._check(self._lib.npbmz_run_phase(self._h, int(phase), m), "npbmz_run_phase")
+
+    def phase_times(self, nsteps):
+        """Seconds spent in each phase (exch, rhs, x, y, z[+add], add) summed
+        over nsteps batched timesteps, from HIP events on the context's stream."""
+        ms = np.zeros(6)
+        self._check(self._lib.npbmz_phase_times(self._h, int(nsteps), _dptr(ms)),
+                    "npbmz_phase_times")
+        return ms * 1e-3
+
+    def norms(self):
+        xcr = np.zeros(5)
+        xce = np.zeros(5)
+        xci = np.zeros(1)
+        self._check(self._lib.npbmz_norms(self._h, _dptr(xcr), _dptr(xce), _dptr(xci)),
+                    "npbmz_norms")
+        return xcr, xce, float(xci[0])
+
+    def zone_norms(self):
+        n = self.zones.count
+        r = np.zeros((n, 5))
+        e = np.zeros((n, 5))
+        self._check(self._lib.npbmz_zone_norms(self._h, _dptr(r), _dptr(e)), "npbmz_zone_norms")
+        return r, e
+
+    def tune(self, budget_ms=2000):
+        self._check(self._lib.npbmz_tune(self._h, int(budget_ms)), "npbmz_tune")
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        self._check(self._lib.npbmz_describe(self._h, buf, 512), "npbmz_describe")
+        return buf.value.decode()

@@ -535,12 +535,14 @@ inline void jacobians(int dir, const Consts& K, T u1, T u2, T u3, T u4, T u5, T 
 // ---------------------------------------------------------------- solves
 // x_solve_cell/x_backsubstitute (x_solve.jl:444-1745), y_solve_cell
 // (y_solve.jl:443-821), z_solve_cell (z_solve.jl:438-864) for FIRST = LAST = 1.
+// Assemble the block rows of one line (transverse indices a, b) into the
+// scratch s and lhsc: fjac/njac for every point, lhsa/lhsb/lhsc for rows
+// 1..size-1 (x_solve.jl:536-708 and its y/z twins).
 template <class T>
-void solve_dir(Zone<T>& z, const Consts& K, int dir) {
+void assemble_line(Zone<T>& z, const Consts& K, int dir, int a, int b, LineScratch<T>& s) {
   const int n[3] = {z.nx, z.ny, z.nz};
   const int nl = n[dir];
-  const int size = nl - 1;  // isize / jsize / ksize
-  LineScratch<T> s(nl);
+  const int size = nl - 1;
   const double d1 = dir == 0 ? K.tx1 : dir == 1 ? K.ty1 : K.tz1;
   const double d2 = dir == 0 ? K.tx2 : dir == 1 ? K.ty2 : K.tz2;
   const double dd[5] = {dir == 0 ? K.dx1 : dir == 1 ? K.dy1 : K.dz1,
@@ -548,51 +550,77 @@ void solve_dir(Zone<T>& z, const Consts& K, int dir) {
                         dir == 0 ? K.dx3 : dir == 1 ? K.dy3 : K.dz3,
                         dir == 0 ? K.dx4 : dir == 1 ? K.dy4 : K.dz4,
                         dir == 0 ? K.dx5 : dir == 1 ? K.dy5 : K.dz5};
-  // lhsabinit(lhsa, lhsb, size): initialize.jl:233-246
+  int da, db;
+  if (dir == 0) { da = 2; db = 1; }
+  else if (dir == 1) { da = 2; db = 0; }
+  else { da = 1; db = 0; }
+  int idx[3];
+  idx[da] = a; idx[db] = b;
+  for (int l = 0; l <= nl - 1; ++l) {
+    idx[dir] = l;
+    const int i = idx[0], j = idx[1], k = idx[2];
+    T u1 = z.U(0, i, j, k);
+    T ri = dir == 0 ? z.rho_i[z.didx(i, j, k)] : T(1.0 / u1);
+    jacobians<T>(dir, K, u1, z.U(1, i, j, k), z.U(2, i, j, k), z.U(3, i, j, k),
+                 z.U(4, i, j, k), ri, z.qs[z.didx(i, j, k)],
+                 [&](int m, int nn) -> T& { return s.f(m, nn, l); },
+                 [&](int m, int nn) -> T& { return s.nj(m, nn, l); });
+  }
+  for (int l = 1; l <= size - 1; ++l) {
+    idx[dir] = l;
+    const int i = idx[0], j = idx[1], k = idx[2];
+    T tmp1 = K.dt * d1;
+    T tmp2 = K.dt * d2;
+    for (int m = 0; m < 5; ++m)
+      for (int nn = 0; nn < 5; ++nn) {
+        T av = -tmp2 * s.f(m, nn, l - 1) - tmp1 * s.nj(m, nn, l - 1);
+        if (m == nn) av = av - tmp1 * dd[m];
+        s.a(m, nn, l) = av;
+        T bv = tmp1 * 2.0 * s.nj(m, nn, l);
+        if (m == nn) bv = 1.0 + tmp1 * 2.0 * s.nj(m, nn, l) + tmp1 * 2.0 * dd[m];
+        s.b(m, nn, l) = bv;
+        T cv = tmp2 * s.f(m, nn, l + 1) - tmp1 * s.nj(m, nn, l + 1);
+        if (m == nn) cv = cv - tmp1 * dd[m];
+        z.LC(m, nn, i, j, k) = cv;
+      }
+  }
+}
+
+inline void transverse(int dir, int& da, int& db) {
+  if (dir == 0) { da = 2; db = 1; }
+  else if (dir == 1) { da = 2; db = 0; }
+  else { da = 1; db = 0; }
+}
+
+// lhsabinit(lhsa, lhsb, size): initialize.jl:233-246
+template <class T>
+void lhsabinit(LineScratch<T>& s, int size) {
   for (int i = 0; i <= size; ++i)
     for (int m = 0; m < 5; ++m) {
       for (int nn = 0; nn < 5; ++nn) { s.a(m, nn, i) = T(0.0); s.b(m, nn, i) = T(0.0); }
       s.b(m, m, i) = T(1.0);
     }
-  int da, db;  // outer, inner transverse loops as in the reference
-  if (dir == 0) { da = 2; db = 1; }
-  else if (dir == 1) { da = 2; db = 0; }
-  else { da = 1; db = 0; }
+}
+
+// x_solve_cell/x_backsubstitute (x_solve.jl:444-1745), y_solve_cell
+// (y_solve.jl:443-821), z_solve_cell (z_solve.jl:438-864) for FIRST = LAST = 1.
+template <class T>
+void solve_dir(Zone<T>& z, const Consts& K, int dir) {
+  const int n[3] = {z.nx, z.ny, z.nz};
+  const int nl = n[dir];
+  const int size = nl - 1;  // isize / jsize / ksize
+  LineScratch<T> s(nl);
+  lhsabinit(s, size);
+  int da, db;
+  transverse(dir, da, db);
   for (int a = 1; a <= n[da] - 2; ++a)
     for (int b = 1; b <= n[db] - 2; ++b) {
       int idx[3];
       idx[da] = a; idx[db] = b;
+      assemble_line(z, K, dir, a, b, s);
       auto at = [&](int l, int& i, int& j, int& k) {
         idx[dir] = l; i = idx[0]; j = idx[1]; k = idx[2];
       };
-      for (int l = 0; l <= nl - 1; ++l) {
-        int i, j, k;
-        at(l, i, j, k);
-        T u1 = z.U(0, i, j, k);
-        T ri = dir == 0 ? z.rho_i[z.didx(i, j, k)] : T(1.0 / u1);
-        jacobians<T>(dir, K, u1, z.U(1, i, j, k), z.U(2, i, j, k), z.U(3, i, j, k),
-                     z.U(4, i, j, k), ri, z.qs[z.didx(i, j, k)],
-                     [&](int m, int nn) -> T& { return s.f(m, nn, l); },
-                     [&](int m, int nn) -> T& { return s.nj(m, nn, l); });
-      }
-      for (int l = 1; l <= size - 1; ++l) {
-        int i, j, k;
-        at(l, i, j, k);
-        T tmp1 = K.dt * d1;
-        T tmp2 = K.dt * d2;
-        for (int m = 0; m < 5; ++m)
-          for (int nn = 0; nn < 5; ++nn) {
-            T av = -tmp2 * s.f(m, nn, l - 1) - tmp1 * s.nj(m, nn, l - 1);
-            if (m == nn) av = av - tmp1 * dd[m];
-            s.a(m, nn, l) = av;
-            T bv = tmp1 * 2.0 * s.nj(m, nn, l);
-            if (m == nn) bv = 1.0 + tmp1 * 2.0 * s.nj(m, nn, l) + tmp1 * 2.0 * dd[m];
-            s.b(m, nn, l) = bv;
-            T cv = tmp2 * s.f(m, nn, l + 1) - tmp1 * s.nj(m, nn, l + 1);
-            if (m == nn) cv = cv - tmp1 * dd[m];
-            z.LC(m, nn, i, j, k) = cv;
-          }
-      }
       auto C = [&](int l) {
         int i, j, k;
         at(l, i, j, k);
@@ -872,6 +900,44 @@ int oracle_bt_consts(void* h, double* out, int cap) {
   const double* src = (const double*)&r->K;
   for (int i = 0; i < n && i < cap; ++i) out[i] = src[i];
   return n;
+}
+
+// ---- test hooks -----------------------------------------------------------
+// binvcrhs / binvrhs of solve_subs.jl on one block system; B, C row-major.
+void oracle_binvcrhs(double* B, double* C, double* r, int with_c) {
+  binvcrhs<double>([B](int m, int n) -> double& { return B[m * 5 + n]; },
+                   [C](int m, int n) -> double& { return C[m * 5 + n]; },
+                   [r](int m) -> double& { return r[m]; }, with_c != 0);
+}
+
+// The block rows of line (a, b) of zone z in direction dir as the solver
+// assembles them (rows 0..nl-1; rows 0 and nl-1 are the identity rows of
+// lhsabinit with zero couplings).  Uses the derived fields of the last
+// compute_rhs.  A, B, C: nl x 25 row-major blocks.
+void oracle_bt_line_blocks(void* h, int z, int dir, int a, int b, double* A, double* B,
+                           double* C) {
+  Run* r = (Run*)h;
+  Zone<double>& zz = r->Z[z];
+  const int n[3] = {zz.nx, zz.ny, zz.nz};
+  const int nl = n[dir];
+  LineScratch<double> s(nl);
+  lhsabinit(s, nl - 1);
+  int da, db;
+  transverse(dir, da, db);
+  // keep lhsc intact: assemble into a copy of the zone
+  Zone<double> tmp = zz;
+  assemble_line(tmp, r->K, dir, a, b, s);
+  int idx[3];
+  idx[da] = a; idx[db] = b;
+  for (int l = 0; l < nl; ++l) {
+    idx[dir] = l;
+    for (int m = 0; m < 5; ++m)
+      for (int q = 0; q < 5; ++q) {
+        A[l * 25 + m * 5 + q] = s.a(m, q, l);
+        B[l * 25 + m * 5 + q] = s.b(m, q, l);
+        C[l * 25 + m * 5 + q] = (l >= 1 && l <= nl - 2) ? tmp.LC(m, q, idx[0], idx[1], idx[2]) : 0.0;
+      }
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,177 @@
+"""BT-MZ on the GPU against the CPU oracle, through the C ABI.
+
+Strict-FP kernels (no FMA contraction) must reproduce the oracle bit for bit:
+initial state, every phase, whole runs.  The FMA kernels (the product's
+default) must keep the norms within 1e-12 per step and 1e-10 at the end of a
+run, and every run must pass the reference's verification table.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleBT
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(P, cls, strict):
+    zones, p = P.zones_for("BT", cls)
+    return P.Context("BT", zones, p.dt, strict=strict), zones, p
+
+
+def _load_oracle_state(ctx, o, zones):
+    for z in range(zones.count):
+        ctx.put_dense(z, 0, o.get(z, OracleBT.U))
+        ctx.put_dense(z, 1, o.get(z, OracleBT.FORCING))
+
+
+def _interior(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("cls", ["S", "W", "A"])
+def test_init_exact_is_bit_exact(gpu, cls):
+    """initialize + exact_rhs on the device equal the host routines."""
+    ctx, zones, p = _ctx(gpu, cls, True)
+    o = OracleBT.for_class(p)
+    ctx.init_exact()
+    for z in range(zones.count):
+        assert np.array_equal(ctx.get_dense(z, 0), o.get(z, OracleBT.U)), z
+        assert np.array_equal(ctx.get_dense(z, 1), o.get(z, OracleBT.FORCING)), z
+    ctx.close()
+
+
+@pytest.mark.parametrize("cls", ["S", "W"])
+@pytest.mark.parametrize("mode", ["perzone", "batched"])
+def test_each_phase_is_bit_exact(gpu, cls, mode):
+    """exch_qbc, compute_rhs, x/y/z_solve and add, one at a time, from the
+    same state as the oracle (a few steps into the run)."""
+    ctx, zones, p = _ctx(gpu, cls, True)
+    o = OracleBT.for_class(p)
+    o.step(3)
+    _load_oracle_state(ctx, o, zones)
+    N = gpu.native
+    phases = [N.PHASE_EXCH, N.PHASE_RHS, N.PHASE_XSOLVE, N.PHASE_YSOLVE, N.PHASE_ZSOLVE,
+              N.PHASE_ADD]
+    for ph in phases:
+        o.phase(ph)
+        ctx.run_phase(ph, mode)
+        for z in range(zones.count):
+            if ph in (N.PHASE_EXCH, N.PHASE_ADD):
+                assert np.array_equal(ctx.get_dense(z, 0), o.get(z, OracleBT.U)), (ph, z)
+            else:
+                g = _interior(ctx.get_dense(z, 2))
+                r = _interior(o.get(z, OracleBT.RHS))
+                assert np.array_equal(g, r), (ph, z, np.abs(g - r).max())
+    ctx.close()
+
+
+def test_full_run_strict_is_bit_exact_class_S(gpu):
+    """60 timesteps of class S: u identical to the oracle's in every zone;
+    norms agree to the last bits of a differently ordered sum."""
+    ctx, zones, p = _ctx(gpu, "S", True)
+    o = OracleBT.for_class(p)
+    ctx.init_exact()
+    ctx.step(p.niter, "graph")
+    o.step(p.niter)
+    for z in range(zones.count):
+        assert np.array_equal(ctx.get_dense(z, 0), o.get(z, OracleBT.U)), z
+    gxcr, gxce, _ = ctx.norms()
+    oxcr, oxce = o.norms()
+    assert np.abs(gxcr / oxcr - 1).max() < 1e-13
+    assert np.abs(gxce / oxce - 1).max() < 1e-13
+    ctx.close()
+
+
+def test_modes_agree_bitwise(gpu):
+    """Per-zone launches, batched launches and the hipGraph replay run the
+    same kernels on the same data: identical results."""
+    res = []
+    for mode in ("perzone", "batched", "graph"):
+        ctx, zones, p = _ctx(gpu, "W", False)
+        ctx.init_exact()
+        ctx.step(10, mode)
+        res.append([ctx.get_dense(z, 0) for z in range(zones.count)])
+        ctx.close()
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("cls", ["S", "W"])
+def test_fma_kernels_track_oracle_per_step(gpu, cls):
+    """FMA kernels: rsdnrm/errnorm within 1e-12 relative of the oracle after
+    every one of the first 10 steps."""
+    ctx, zones, p = _ctx(gpu, cls, False)
+    o = OracleBT.for_class(p)
+    ctx.init_exact()
+    for s in range(10):
+        ctx.step(1, "batched")
+        o.step(1)
+        gxcr, gxce, _ = ctx.norms()
+        oxcr, oxce = o.norms()
+        assert np.abs(gxcr / oxcr - 1).max() < 1e-12, (s, gxcr, oxcr)
+        assert np.abs(gxce / oxce - 1).max() < 1e-12, (s, gxce, oxce)
+    ctx.close()
+
+
+@pytest.mark.parametrize("cls", ["S", "W", "A"])
+def test_fma_run_verifies_and_tracks_oracle(gpu, cls):
+    """A whole class run with the FMA kernels: the reference's verification
+    passes and the end-of-run norms are within 1e-10 of the oracle."""
+    r = gpu.go("BT", cls, mode="graph", verbose=False)
+    assert r["verified"], r["rel"]
+    zones, p = gpu.zones_for("BT", cls)
+    o = OracleBT.for_class(p)
+    o.step(p.niter)
+    oxcr, oxce = o.norms()
+    assert np.abs(np.array(r["xcr"]) / oxcr - 1).max() < 1e-10
+    assert np.abs(np.array(r["xce"]) / oxce - 1).max() < 1e-10
+
+
+def test_class_B_headline_config(gpu):
+    """BT-MZ 304x208x17 in 8x8 uneven zones: two strict steps bit-identical to
+    the oracle, then the full 200-step run verifies against the table."""
+    ctx, zones, p = _ctx(gpu, "B", True)
+    o = OracleBT.for_class(p)
+    ctx.init_exact()
+    ctx.step(2, "graph")
+    o.step(2)
+    for z in range(zones.count):
+        assert np.array_equal(ctx.get_dense(z, 0), o.get(z, OracleBT.U)), z
+    ctx.close()
+    r = gpu.go("BT", "B", mode="graph", verbose=False)
+    assert r["verified"], r["rel"]
+
+
+def test_put_get_julia_layout(gpu):
+    """put/get through the reference's own OffsetArray layout of BT u
+    (1:5, -2:nx+2, -2:ny+2, -2:nz+2), touching only the grid points."""
+    ctx, zones, p = _ctx(gpu, "S", False)
+    z = 1
+    nx, ny, nz = zones.nx[z], zones.ny[z], zones.nz[z]
+    lo, hi = (1, -2, -2, -2), (5, nx + 2, ny + 2, nz + 2)
+    rng = np.random.default_rng(3)
+    host = rng.standard_normal((nz + 5, ny + 5, nx + 5, 5))
+    ctx.put_field(z, 0, host, lo, hi)
+    back = np.full_like(host, -7.0)
+    ctx.get_field(z, 0, back, lo, hi)
+    grid = (slice(2, nz + 2), slice(2, ny + 2), slice(2, nx + 2))
+    assert np.array_equal(back[grid], host[grid])
+    mask = np.ones(host.shape[:3], bool)
+    mask[grid] = False
+    assert np.all(back[mask] == -7.0)
+    assert np.array_equal(ctx.get_dense(z, 0), host[grid])
+    ctx.close()
+
+
+def test_errors_are_reported_not_raised_across_the_abi(gpu):
+    ctx, zones, p = _ctx(gpu, "S", False)
+    with pytest.raises(gpu.NpbmzError, match="zone out of range"):
+        ctx.put_field(99, 0, np.zeros(40), (1, 0, 0, 0), (5, 1, 1, 1))
+    with pytest.raises(gpu.NpbmzError, match="bounds describe"):
+        ctx.put_field(0, 0, np.zeros(10), (1, 0, 0, 0), (5, 1, 1, 1))
+    with pytest.raises(gpu.NpbmzError, match="does not cover"):
+        ctx.put_field(0, 0, np.zeros((2, 2, 2, 5)), (1, 0, 0, 0), (5, 1, 1, 1))
+    with pytest.raises(gpu.NpbmzError, match="unknown mode"):
+        ctx.step(1, 7)
+    ctx.close()
