@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perzone", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="size of the plain copy kernel run for the achievable-bandwidth "
+                         "reference and counter calibration (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -103,6 +106,11 @@ def main():
     ctx.init_exact()
     ph = ctx.phase_times(K)[:5]
     sched = ctx.describe()
+    copy = None
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        tc = ctx.probe_copy(nb, 20)
+        copy = {"bytes_per_launch": 2 * nb, "GBps": round(2 * nb / tc / 1e9, 1)}
     ctx.close()
 
     per_launch_s = {name: ph[i] / K for i, name in enumerate(PHASES)}
@@ -144,7 +152,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes, "copy_kernel": copy},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -151,6 +151,13 @@ int npbmz_describe(const npbmz_ctx* ctx, char* buf, int32_t cap);
 /* Number of zones and total grid points held by ctx. */
 int npbmz_size(const npbmz_ctx* ctx, int32_t* nzones, int64_t* npoints);
 
+/* Diagnostic with no counterpart in the reference: copy nbytes between two
+ * scratch buffers with a plain streaming kernel, reps times, and return the
+ * average kernel time.  Its known traffic (nbytes read + nbytes written per
+ * launch) calibrates profiler byte counters and gives the achievable HBM
+ * bandwidth on the device. */
+int npbmz_probe_copy(npbmz_ctx* ctx, int64_t nbytes, int32_t reps, double* ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

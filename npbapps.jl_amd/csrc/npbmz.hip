@@ -73,14 +73,24 @@ __global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
   for (int m = 0; m < 5; ++m) d[m] = s[m];
 }
 
+// Streaming copy used to calibrate byte counters (npbmz_probe_copy).
+__global__ __launch_bounds__(256) void copy_probe_kernel(const double4* __restrict__ src,
+                                                         double4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 namespace exact {
 #pragma clang fp contract(off)
 #include "bt_kernels.inc"
+#include "sp_kernels.inc"
 }  // namespace exact
 
 namespace fast {
 #pragma clang fp contract(fast)
 #include "bt_kernels.inc"
+#include "sp_kernels.inc"
 }  // namespace fast
 
 }  // namespace npbmz
@@ -251,7 +261,10 @@ int build_tables(npbmz_ctx* ctx) {
     int slots = c.t_line[d].n;
     for (int z = 0; z < c.nz_; ++z) slots = std::max(slots, c.t_line[d].zcount[z]);
     if (c.solve_grid[d] > 0) slots = std::max(slots, c.solve_grid[d]);
-    slot = std::max(slot, (size_t)slots * mr * 25 * 64);
+    // reduced coefficients kept per row for the back sweep: BT's 5x5 C'
+    // block, SP's lhs(4:5) of three systems
+    const int per_row = c.bench == NPBMZ_SP ? 6 : 25;
+    slot = std::max(slot, (size_t)slots * mr * per_row * 64);
   }
   if (slot > c.cscr_slot) {
     if (c.cscr) (void)hipFree(c.cscr);
@@ -263,10 +276,14 @@ int build_tables(npbmz_ctx* ctx) {
   std::vector<Face> fh;
   for (int z = 0; z < c.nz_; ++z) {
     int nxf = (c.ny[z] - 2) * (c.nz[z] - 2), nyf = (c.nx[z] - 2) * (c.nz[z] - 2);
+    // SP's exchange fills j = 0 from the north neighbour and j = ny-1 from
+    // the south one (src/SP/exchange_qbc.jl:247-295, 409-455), the mirror of
+    // BT's (src/BT/exchange_qbc.jl:247-295, 409-455)
+    const bool yswap = c.bench == NPBMZ_SP;
     fh.push_back(Face{z, c.west[z], 0, nxf});
     fh.push_back(Face{z, c.east[z], 1, nxf});
-    fh.push_back(Face{z, c.south[z], 2, nyf});
-    fh.push_back(Face{z, c.north[z], 3, nyf});
+    fh.push_back(Face{z, yswap ? c.north[z] : c.south[z], 2, nyf});
+    fh.push_back(Face{z, yswap ? c.south[z] : c.north[z], 3, nyf});
   }
   if (!c.faces) CK(hipMalloc(&c.faces, sizeof(Face) * fh.size()));
   CK(hipMemcpy(c.faces, fh.data(), sizeof(Face) * fh.size(), hipMemcpyHostToDevice));
@@ -286,33 +303,33 @@ int build_tables(npbmz_ctx* ctx) {
 bool strict(const npbmz_ctx* c) { return (c->flags & NPBMZ_FLAG_STRICT_FP) != 0; }
 
 // ------------------------------------------------------------ launches
-// Launch `kern` over table t: all of it (zone < 0) or one zone's range.
-template <class Kern, class... Args>
-int launch_pts(npbmz_ctx* ctx, Kern kern, const Table& t, int zone, int nt, Args... args) {
-  int first = zone < 0 ? 0 : t.zfirst[zone];
-  int n = zone < 0 ? t.n : t.zcount[zone];
-  if (n <= 0) return NPBMZ_OK;
-  hipLaunchKernelGGL(kern, dim3(n), dim3(nt), 0, ctx->stream, args..., t.d + first);
-  CK(hipGetLastError());
-  return NPBMZ_OK;
-}
+// Every launcher covers either a whole task table (zone < 0, the batched
+// form) or one zone's range of it (the per-zone form).
 
-int bt_rhs(npbmz_ctx* ctx, int zone) {
+// compute_rhs.  For SP the timestep's rhs phase is compute_rhs + txinvr;
+// plain = true gives compute_rhs alone (SP's verify, src/SP/verify.jl:436).
+int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false) {
   auto& c = *ctx;
   const Table& t = c.t_int;
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
   if (n <= 0) return NPBMZ_OK;
-  if (strict(ctx))
-    hipLaunchKernelGGL(exact::bt::rhs_kernel<kPtNT>, dim3(n), dim3(kPtNT), 0, c.stream, c.u,
-                       c.frc, c.rhs, c.zd, t.d + first, c.K);
-  else
-    hipLaunchKernelGGL(fast::bt::rhs_kernel<kPtNT>, dim3(n), dim3(kPtNT), 0, c.stream, c.u,
-                       c.frc, c.rhs, c.zd, t.d + first, c.K);
+  const bool tx = c.bench == NPBMZ_SP && !plain;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, c.u, c.frc, c.rhs, c.zd,
+                       t.d + first, c.K);
+  };
+  if (strict(ctx)) {
+    if (tx) go(exact::bt::rhs_kernel<kPtNT, true>);
+    else go(exact::bt::rhs_kernel<kPtNT, false>);
+  } else {
+    if (tx) go(fast::bt::rhs_kernel<kPtNT, true>);
+    else go(fast::bt::rhs_kernel<kPtNT, false>);
+  }
   CK(hipGetLastError());
   return NPBMZ_OK;
 }
 
-int bt_add(npbmz_ctx* ctx, int zone) {
+int k_add(npbmz_ctx* ctx, int zone) {
   auto& c = *ctx;
   const Table& t = c.t_int;
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
@@ -324,27 +341,34 @@ int bt_add(npbmz_ctx* ctx, int zone) {
 }
 
 template <int DIR, bool FUSE>
-int bt_solve_t(npbmz_ctx* ctx, int zone) {
+int k_solve_t(npbmz_ctx* ctx, int zone) {
   auto& c = *ctx;
   const Table& t = c.t_line[DIR];
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
   if (n <= 0) return NPBMZ_OK;
   int grid = n;
   if (zone < 0 && c.solve_grid[DIR] > 0) grid = std::min(n, c.solve_grid[DIR]);
-  if (strict(ctx))
-    hipLaunchKernelGGL((exact::bt::solve_kernel<DIR, FUSE>), dim3(grid), dim3(kLineNT), 0,
-                       c.stream, c.u, c.rhs, c.cscr, c.zd, t.d + first, n, c.maxrows[DIR], c.K);
-  else
-    hipLaunchKernelGGL((fast::bt::solve_kernel<DIR, FUSE>), dim3(grid), dim3(kLineNT), 0,
-                       c.stream, c.u, c.rhs, c.cscr, c.zd, t.d + first, n, c.maxrows[DIR], c.K);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kLineNT), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
+                       t.d + first, n, c.maxrows[DIR], c.K);
+  };
+  if (c.bench == NPBMZ_SP) {
+    if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>);
+    else go(fast::sp::solve_kernel<DIR, FUSE>);
+  } else {
+    if (strict(ctx)) go(exact::bt::solve_kernel<DIR, FUSE>);
+    else go(fast::bt::solve_kernel<DIR, FUSE>);
+  }
   CK(hipGetLastError());
   return NPBMZ_OK;
 }
 
-int bt_solve(npbmz_ctx* ctx, int dir, bool fuse, int zone) {
-  if (dir == 0) return bt_solve_t<0, false>(ctx, zone);
-  if (dir == 1) return bt_solve_t<1, false>(ctx, zone);
-  return fuse ? bt_solve_t<2, true>(ctx, zone) : bt_solve_t<2, false>(ctx, zone);
+// x/y/z_solve (BT block tridiagonal, SP pentadiagonal with its inverse
+// transforms); fuse folds add into the z sweep.
+int k_solve(npbmz_ctx* ctx, int dir, bool fuse, int zone) {
+  if (dir == 0) return k_solve_t<0, false>(ctx, zone);
+  if (dir == 1) return k_solve_t<1, false>(ctx, zone);
+  return fuse ? k_solve_t<2, true>(ctx, zone) : k_solve_t<2, false>(ctx, zone);
 }
 
 int exch(npbmz_ctx* ctx) {
@@ -363,22 +387,22 @@ int exch(npbmz_ctx* ctx) {
 // per-zone order (all zones' rhs, then all x solves, ...) is equivalent to
 // the reference's zone-by-zone adi because zones are independent after the
 // exchange.
-int bt_step(npbmz_ctx* ctx, int mode) {
+int adi_step(npbmz_ctx* ctx, int mode) {
   int rc;
   if ((rc = exch(ctx))) return rc;
   if (mode == NPBMZ_MODE_PERZONE) {
     for (int z = 0; z < ctx->nz_; ++z) {
-      if ((rc = bt_rhs(ctx, z))) return rc;
-      if ((rc = bt_solve(ctx, 0, false, z))) return rc;
-      if ((rc = bt_solve(ctx, 1, false, z))) return rc;
-      if ((rc = bt_solve(ctx, 2, true, z))) return rc;
+      if ((rc = k_rhs(ctx, z))) return rc;
+      if ((rc = k_solve(ctx, 0, false, z))) return rc;
+      if ((rc = k_solve(ctx, 1, false, z))) return rc;
+      if ((rc = k_solve(ctx, 2, true, z))) return rc;
     }
     return NPBMZ_OK;
   }
-  if ((rc = bt_rhs(ctx, -1))) return rc;
-  if ((rc = bt_solve(ctx, 0, false, -1))) return rc;
-  if ((rc = bt_solve(ctx, 1, false, -1))) return rc;
-  if ((rc = bt_solve(ctx, 2, true, -1))) return rc;
+  if ((rc = k_rhs(ctx, -1))) return rc;
+  if ((rc = k_solve(ctx, 0, false, -1))) return rc;
+  if ((rc = k_solve(ctx, 1, false, -1))) return rc;
+  if ((rc = k_solve(ctx, 2, true, -1))) return rc;
   return NPBMZ_OK;
 }
 
@@ -389,11 +413,11 @@ int run_phase(npbmz_ctx* ctx, int phase, int mode) {
   if (phase == NPBMZ_PHASE_EXCH) return exch(ctx);
   for (int z = zlo; z < zhi; ++z) {
     switch (phase) {
-      case NPBMZ_PHASE_RHS: rc = bt_rhs(ctx, z); break;
-      case NPBMZ_PHASE_XSOLVE: rc = bt_solve(ctx, 0, false, z); break;
-      case NPBMZ_PHASE_YSOLVE: rc = bt_solve(ctx, 1, false, z); break;
-      case NPBMZ_PHASE_ZSOLVE: rc = bt_solve(ctx, 2, false, z); break;
-      case NPBMZ_PHASE_ADD: rc = bt_add(ctx, z); break;
+      case NPBMZ_PHASE_RHS: rc = k_rhs(ctx, z); break;
+      case NPBMZ_PHASE_XSOLVE: rc = k_solve(ctx, 0, false, z); break;
+      case NPBMZ_PHASE_YSOLVE: rc = k_solve(ctx, 1, false, z); break;
+      case NPBMZ_PHASE_ZSOLVE: rc = k_solve(ctx, 2, false, z); break;
+      case NPBMZ_PHASE_ADD: rc = k_add(ctx, z); break;
       default: return fail(ctx, NPBMZ_EINVAL, "unknown phase");
     }
     if (rc) return rc;
@@ -405,7 +429,7 @@ int ensure_graph(npbmz_ctx* ctx) {
   auto& c = *ctx;
   if (c.gexec) return NPBMZ_OK;
   CK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  int rc = bt_step(ctx, NPBMZ_MODE_BATCHED);
+  int rc = adi_step(ctx, NPBMZ_MODE_BATCHED);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c.stream, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -435,8 +459,8 @@ const char* npbmz_last_error(const npbmz_ctx* ctx) {
 int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   if (!p || !out) return fail(nullptr, NPBMZ_EINVAL, "null argument");
   *out = nullptr;
-  if (p->bench != NPBMZ_BT)
-    return fail(nullptr, NPBMZ_ENOTSUP, "only BT (bench 0) is built in this library");
+  if (p->bench != NPBMZ_BT && p->bench != NPBMZ_SP)
+    return fail(nullptr, NPBMZ_ENOTSUP, "bench must be NPBMZ_BT or NPBMZ_SP");
   if (p->nzones <= 0 || p->nzones != p->x_zones * p->y_zones)
     return fail(nullptr, NPBMZ_EINVAL, "nzones must equal x_zones*y_zones and be positive");
   if (!p->nx || !p->ny || !p->nz || !p->iz_west || !p->iz_east || !p->iz_south || !p->iz_north)
@@ -446,8 +470,10 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     return fail(nullptr, NPBMZ_EINVAL, "grid sizes must be at least 2");
   int n = p->nzones;
   for (int z = 0; z < n; ++z) {
-    if (p->nx[z] < 3 || p->ny[z] < 3 || p->nz[z] < 3)
-      return fail(nullptr, NPBMZ_EINVAL, "zone dimension below 3 (make_set.jl:100-104)");
+    // below 5 points the reference's dissipation stencils read its halo
+    // values (rhs.jl:158-210, lhsx.jl:48-90), which this library does not keep
+    if (p->nx[z] < 5 || p->ny[z] < 5 || p->nz[z] < 5)
+      return fail(nullptr, NPBMZ_EINVAL, "zone dimension below 5");
     if ((int64_t)p->nx[z] * p->ny[z] * p->nz[z] > (1ll << 30))
       return fail(nullptr, NPBMZ_EINVAL, "zone too large");
     int nb[4] = {p->iz_west[z], p->iz_east[z], p->iz_south[z], p->iz_north[z]};
@@ -658,7 +684,7 @@ int npbmz_step(npbmz_ctx* ctx, int32_t nsteps, int32_t mode, double* elapsed_ms)
   for (int s = 0; s < nsteps; ++s) {
     if (mode == NPBMZ_MODE_GRAPH) {
       CK(hipGraphLaunch(c.gexec, c.stream));
-    } else if ((rc = bt_step(ctx, mode))) {
+    } else if ((rc = adi_step(ctx, mode))) {
       return rc;
     }
   }
@@ -694,8 +720,8 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
       hipEvent_t* e = &ev[((size_t)s * nph + ph) * 2];
       if (hipEventRecord(e[0], c.stream) != hipSuccess) { rc = fail(ctx, NPBMZ_EHIP, "record"); break; }
       if (ph == 0) rc = exch(ctx);
-      else if (ph == 1) rc = bt_rhs(ctx, -1);
-      else rc = bt_solve(ctx, ph - 2, true, -1);
+      else if (ph == 1) rc = k_rhs(ctx, -1);
+      else rc = k_solve(ctx, ph - 2, true, -1);
       if (rc == NPBMZ_OK && hipEventRecord(e[1], c.stream) != hipSuccess)
         rc = fail(ctx, NPBMZ_EHIP, "record");
     }
@@ -738,7 +764,7 @@ int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci) {
   CK(hipMemcpyAsync(pe.data(), c.partial, pe.size() * sizeof(double), hipMemcpyDeviceToHost,
                     c.stream));
   // compute_rhs without a new exchange, then rhs_norm
-  int rc = bt_rhs(ctx, -1);
+  int rc = k_rhs(ctx, -1, true);
   if (rc) return rc;
   hipLaunchKernelGGL(exact::bt::norm_kernel<kPtNT>, dim3(c.t_int.n), dim3(kPtNT), 0, c.stream,
                      c.u, c.rhs, c.zd, c.t_int.d, c.partial, 1, c.K);
@@ -796,6 +822,34 @@ int npbmz_size(const npbmz_ctx* ctx, int32_t* nzones, int64_t* npoints) {
   if (nzones) *nzones = ctx->nz_;
   if (npoints) *npoints = ctx->npoints;
   return NPBMZ_OK;
+}
+
+int npbmz_probe_copy(npbmz_ctx* ctx, int64_t nbytes, int32_t reps, double* ms_per_launch) {
+  if (!ctx) return NPBMZ_EINVAL;
+  if (nbytes < 32 || nbytes > (int64_t(16) << 30) || reps <= 0 || !ms_per_launch)
+    return fail(ctx, NPBMZ_EINVAL, "nbytes must be in [32, 16 GiB], reps > 0");
+  auto& c = *ctx;
+  const int64_t n = nbytes / 32;
+  double4 *a = nullptr, *b = nullptr;
+  CK(hipMalloc(&a, n * 32));
+  if (hipMalloc(&b, n * 32) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, NPBMZ_ENOMEM, "probe buffer");
+  }
+  int rc = NPBMZ_OK;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, c.stream, a, b, n);
+  (void)hipEventRecord(c.ev0, c.stream);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, c.stream, a, b, n);
+  (void)hipEventRecord(c.ev1, c.stream);
+  float ms = 0.f;
+  if (hipEventSynchronize(c.ev1) != hipSuccess || hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess)
+    rc = fail(ctx, NPBMZ_EHIP, "probe timing");
+  *ms_per_launch = ms / reps;
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }
 
 }  // extern "C"

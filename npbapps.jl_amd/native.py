@@ -27,6 +27,7 @@ EXPORTS = (
     "npbmz_create", "npbmz_destroy", "npbmz_last_error", "npbmz_put_field",
     "npbmz_get_field", "npbmz_init_exact", "npbmz_step", "npbmz_run_phase", "npbmz_phase_times",
     "npbmz_norms", "npbmz_zone_norms", "npbmz_tune", "npbmz_describe", "npbmz_size",
+    "npbmz_probe_copy",
 )
 
 
@@ -78,6 +79,7 @@ def load(path=LIB_PATH):
     lib.npbmz_tune.argtypes = [P, i32]
     lib.npbmz_describe.argtypes = [P, ctypes.c_char_p, i32]
     lib.npbmz_size.argtypes = [P, ctypes.POINTER(i32), i64p]
+    lib.npbmz_probe_copy.argtypes = [P, i64, i32, dp]
     for f in EXPORTS:
         if f not in ("npbmz_destroy", "npbmz_last_error"):
             getattr(lib, f).restype = ctypes.c_int
@@ -217,6 +219,13 @@ class Context:
 
     def tune(self, budget_ms=2000):
         self._check(self._lib.npbmz_tune(self._h, int(budget_ms)), "npbmz_tune")
+
+    def probe_copy(self, nbytes, reps=20):
+        """Seconds per launch of a plain nbytes device-to-device copy kernel."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.npbmz_probe_copy(self._h, int(nbytes), int(reps), ctypes.byref(ms)),
+                    "npbmz_probe_copy")
+        return ms.value * 1e-3
 
     def describe(self):
         buf = ctypes.create_string_buffer(512)

@@ -29,26 +29,29 @@ def load():
     L = ctypes.CDLL(ORACLE_LIB)
     vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     dp = ctypes.POINTER(d)
-    L.oracle_bt_create.restype = vp
-    L.oracle_bt_create.argtypes = [i, i, i, i, i, d, d]
-    L.oracle_bt_destroy.argtypes = [vp]
-    L.oracle_bt_destroy.restype = None
-    L.oracle_bt_set_threads.argtypes = [vp, i]
-    L.oracle_bt_nzones.argtypes = [vp]
+    for b in ("bt", "sp"):
+        f = lambda n: getattr(L, f"oracle_{b}_{n}")
+        f("create").restype = vp
+        f("create").argtypes = [i, i, i, i, i, d, d]
+        f("destroy").argtypes = [vp]
+        f("set_threads").argtypes = [vp, i]
+        f("nzones").argtypes = [vp]
+        f("phase").argtypes = [vp, i]
+        f("step").argtypes = [vp, i]
+        f("norms").argtypes = [vp, dp, dp]
+        f("get").argtypes = [vp, i, i, dp]
+        f("set").argtypes = [vp, i, i, dp]
+        for n in ("destroy", "set_threads", "phase", "step", "norms", "get", "set"):
+            f(n).restype = None
     L.oracle_bt_zone.argtypes = [vp, i, ctypes.POINTER(i)]
-    L.oracle_bt_phase.argtypes = [vp, i]
-    L.oracle_bt_step.argtypes = [vp, i]
-    L.oracle_bt_norms.argtypes = [vp, dp, dp]
-    L.oracle_bt_get.argtypes = [vp, i, i, dp]
-    L.oracle_bt_set.argtypes = [vp, i, i, dp]
+    L.oracle_bt_zone.restype = None
     L.oracle_bt_consts.argtypes = [vp, dp, i]
     L.oracle_binvcrhs.argtypes = [dp, dp, dp, i]
     L.oracle_binvcrhs.restype = None
     L.oracle_bt_line_blocks.argtypes = [vp, i, i, i, i, dp, dp, dp]
     L.oracle_bt_line_blocks.restype = None
-    for f in ("oracle_bt_set_threads", "oracle_bt_phase", "oracle_bt_step", "oracle_bt_norms",
-              "oracle_bt_get", "oracle_bt_set", "oracle_bt_zone"):
-        getattr(L, f).restype = None
+    L.oracle_sp_line_lhs.argtypes = [vp, i, i, i, i, dp]
+    L.oracle_sp_line_lhs.restype = None
     _lib = L
     return L
 
@@ -57,23 +60,24 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-class OracleBT:
-    """The BT-MZ CPU restatement for one problem (all zones on one rank)."""
+class _OracleMZ:
+    """A CPU restatement of one multi-zone benchmark (all zones on one rank)."""
 
     U, FORCING, RHS = 0, 1, 2
+    PREFIX = None
 
     def __init__(self, x_zones, y_zones, gx, gy, gz, ratio, dt, threads=None):
         self.L = load()
-        self.h = ctypes.c_void_p(self.L.oracle_bt_create(x_zones, y_zones, gx, gy, gz,
-                                                         float(ratio), float(dt)))
-        self.L.oracle_bt_set_threads(self.h, threads or min(8, os.cpu_count() or 1))
+        self._f = lambda n: getattr(self.L, f"oracle_{self.PREFIX}_{n}")
+        self.h = ctypes.c_void_p(self._f("create")(x_zones, y_zones, gx, gy, gz, float(ratio),
+                                                   float(dt)))
+        self._f("set_threads")(self.h, threads or min(8, os.cpu_count() or 1))
         self.dt = dt
-        n = self.L.oracle_bt_nzones(self.h)
-        z7 = (ctypes.c_int * 7)()
-        self.zones = []
-        for z in range(n):
-            self.L.oracle_bt_zone(self.h, z, z7)
-            self.zones.append(tuple(z7))
+        # zone geometry from the shared zone split (same formula for BT and SP)
+        import npbapps_amd as P
+        zs = P.zone_setup(x_zones, y_zones, gx, gy, gz, ratio)
+        self.zones = [(zs.nx[z], zs.ny[z], zs.nz[z], zs.west[z], zs.east[z], zs.south[z],
+                       zs.north[z]) for z in range(zs.count)]
 
     @classmethod
     def for_class(cls, params, **kw):
@@ -82,7 +86,7 @@ class OracleBT:
 
     def close(self):
         if self.h:
-            self.L.oracle_bt_destroy(self.h)
+            self._f("destroy")(self.h)
             self.h = None
 
     def __del__(self):
@@ -97,24 +101,28 @@ class OracleBT:
     def get(self, z, field):
         nx, ny, nz = self.dims(z)
         out = np.zeros((nz, ny, nx, 5))
-        self.L.oracle_bt_get(self.h, z, field, _dp(out))
+        self._f("get")(self.h, z, field, _dp(out))
         return out
 
     def set(self, z, field, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64)
-        self.L.oracle_bt_set(self.h, z, field, _dp(arr))
+        self._f("set")(self.h, z, field, _dp(arr))
 
     def step(self, n):
-        self.L.oracle_bt_step(self.h, n)
+        self._f("step")(self.h, n)
 
     def phase(self, p):
-        self.L.oracle_bt_phase(self.h, p)
+        self._f("phase")(self.h, p)
 
     def norms(self):
         xcr = np.zeros(5)
         xce = np.zeros(5)
-        self.L.oracle_bt_norms(self.h, _dp(xcr), _dp(xce))
+        self._f("norms")(self.h, _dp(xcr), _dp(xce))
         return xcr, xce
+
+
+class OracleBT(_OracleMZ):
+    PREFIX = "bt"
 
     def line_blocks(self, z, d, a, b):
         nl = self.dims(z)[d]
@@ -123,6 +131,20 @@ class OracleBT:
         C = np.zeros((nl, 5, 5))
         self.L.oracle_bt_line_blocks(self.h, z, d, a, b, _dp(A), _dp(B), _dp(C))
         return A, B, C
+
+
+class OracleSP(_OracleMZ):
+    PREFIX = "sp"
+
+    def line_lhs(self, z, d, a, b):
+        nl = self.dims(z)[d]
+        out = np.zeros((nl, 15))
+        self.L.oracle_sp_line_lhs(self.h, z, d, a, b, _dp(out))
+        return out
+
+
+def oracle_for(bench):
+    return {"BT": OracleBT, "SP": OracleSP}[bench]
 
 
 def binvcrhs(B, C, r, with_c=True):
