@@ -52,6 +52,15 @@ def load():
     L.oracle_bt_line_blocks.restype = None
     L.oracle_sp_line_lhs.argtypes = [vp, i, i, i, i, dp]
     L.oracle_sp_line_lhs.restype = None
+    L.oracle_lu_create.restype = vp
+    L.oracle_lu_create.argtypes = [i, i, i, i, i, d, d, i]
+    for n, a in (("destroy", [vp]), ("set_threads", [vp, i]), ("phase", [vp, i]),
+                 ("step", [vp, i]), ("norms", [vp, dp, dp, dp, dp]), ("get", [vp, i, i, dp]),
+                 ("set", [vp, i, i, dp]), ("zone", [vp, i, ctypes.POINTER(i)]),
+                 ("point_blocks", [vp, i, i, i, i, i, dp]), ("set_exact", [vp, i])):
+        getattr(L, f"oracle_lu_{n}").argtypes = a
+        getattr(L, f"oracle_lu_{n}").restype = None
+    L.oracle_lu_nzones.argtypes = [vp]
     _lib = L
     return L
 
@@ -143,8 +152,81 @@ class OracleSP(_OracleMZ):
         return out
 
 
+class OracleLU:
+    """The CPU restatement of LU-MZ (oracle/lu_oracle.cpp).  flags: bit 0
+    exchanges planes k = 2..nz-1 (NPB-MZ) instead of the reference's
+    k = 1..nz-2; bit 1 uses NPB's signs in buts' back substitution."""
+
+    U, FORCING, RHS = 0, 1, 2
+    NPB_EXCH, NPB_BUTS = 1, 2
+
+    def __init__(self, x_zones, y_zones, gx, gy, gz, dt, omega=1.2, flags=0, threads=None):
+        self.L = load()
+        self.h = ctypes.c_void_p(self.L.oracle_lu_create(x_zones, y_zones, gx, gy, gz, float(dt),
+                                                         float(omega), int(flags)))
+        self.L.oracle_lu_set_threads(self.h, threads or min(8, os.cpu_count() or 1))
+        self.dt, self.omega, self.flags = dt, omega, flags
+        self.zones = []
+        for z in range(self.L.oracle_lu_nzones(self.h)):
+            out = (ctypes.c_int * 7)()
+            self.L.oracle_lu_zone(self.h, z, out)
+            self.zones.append(tuple(out))
+
+    @classmethod
+    def for_class(cls, params, **kw):
+        p = params
+        return cls(p.x_zones, p.y_zones, p.gx, p.gy, p.gz, p.dt, **kw)
+
+    def close(self):
+        if self.h:
+            self.L.oracle_lu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dims(self, z):
+        return self.zones[z][:3]
+
+    def get(self, z, field):
+        nx, ny, nz = self.dims(z)
+        out = np.zeros((nz, ny, nx, 5))
+        self.L.oracle_lu_get(self.h, z, field, _dp(out))
+        return out
+
+    def set(self, z, field, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self.L.oracle_lu_set(self.h, z, field, _dp(arr))
+
+    def step(self, n):
+        self.L.oracle_lu_step(self.h, n)
+
+    def phase(self, p):
+        self.L.oracle_lu_phase(self.h, p)
+
+    def norms(self, per_zone=False):
+        xcr, xce, xci = np.zeros(5), np.zeros(5), np.zeros(1)
+        zo = np.zeros((len(self.zones), 11))
+        self.L.oracle_lu_norms(self.h, _dp(xcr), _dp(xce), _dp(xci), _dp(zo))
+        if per_zone:
+            return xcr, xce, xci[0], zo
+        return xcr, xce, xci[0]
+
+    def point_blocks(self, z, upper, i, j, k):
+        """d and the k-, j-, i-neighbour blocks of 1-based point (i, j, k)."""
+        out = np.zeros((4, 5, 5))
+        self.L.oracle_lu_point_blocks(self.h, z, int(upper), i, j, k, _dp(out))
+        return out
+
+    def set_exact(self, z):
+        self.L.oracle_lu_set_exact(self.h, z)
+
+
 def oracle_for(bench):
-    return {"BT": OracleBT, "SP": OracleSP}[bench]
+    return {"BT": OracleBT, "SP": OracleSP, "LU": OracleLU}[bench]
 
 
 def binvcrhs(B, C, r, with_c=True):
