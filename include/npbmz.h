@@ -42,6 +42,11 @@ extern "C" {
 /* creation flags */
 #define NPBMZ_FLAG_STRICT_FP 1 /* kernels built without fused multiply-add:
                                   bit-identical to the reference's CPU path */
+#define NPBMZ_FLAG_LU_NPB_EXCH 2 /* LU: exchange face planes k = 2..nz-1 (1-based)
+                                    as NPB-MZ does, instead of the reference's
+                                    k = 1..nz-2 (src/LU/exchange_qbc.jl:85,109,
+                                    135,160,205,226,247,268); only this choice
+                                    reproduces src/LU/verify.jl's tables */
 
 /* fields for put/get */
 #define NPBMZ_FIELD_U 0       /* conserved variables u (BT/SP/LU) */
@@ -64,6 +69,10 @@ extern "C" {
 #define NPBMZ_PHASE_YSOLVE 3
 #define NPBMZ_PHASE_ZSOLVE 4
 #define NPBMZ_PHASE_ADD 5
+/* LU's phases (exch and rhs as above) */
+#define NPBMZ_PHASE_SSOR_LOWER 2  /* rsd *= dt, jacld + blts (src/LU/ssor.jl:62-108) */
+#define NPBMZ_PHASE_SSOR_UPPER 3  /* jacu + buts (src/LU/ssor.jl:114-182) */
+#define NPBMZ_PHASE_SSOR_UPDATE 4 /* u += rsd/(omega(2-omega)) (src/LU/ssor.jl:193-202) */
 
 typedef struct npbmz_ctx npbmz_ctx;
 
@@ -137,12 +146,24 @@ int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci);
  * (nzones x 5 each, zone-major); either pointer may be NULL. */
 int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err);
 
-/* Autotune the batched launch schedule (zone order, workgroups per zone,
- * persistent vs one-shot grid) by timing candidate schedules for about
- * budget_ms; the choice is kept in the context and used by the batched and
- * graph modes.  No counterpart in the reference (its zone->thread schedule
- * is Threads.@threads over zones, src/BT/bt-node.jl:318). */
+/* Autotune the batched launch schedule by timing candidate schedules for
+ * about budget_ms in total: for BT/SP the zone order of the task tables
+ * (largest first or natural) and the solve grid (one block per 64 lines, or
+ * a persistent grid of 1024/2048 blocks striding over them); for LU the SSOR
+ * sweep schedule (one launch per i+j+k hyperplane over all zones, or one
+ * workgroup of 128/256/512 threads per zone walking all hyperplanes).  The
+ * state is restored afterwards; the fastest schedule is kept in the context
+ * for the batched and graph modes.  No counterpart in the reference (its
+ * zone->thread schedule is Threads.@threads over zones,
+ * src/BT/bt-node.jl:318). */
 int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
+
+/* Set one schedule knob directly (the tuner's candidates, for tests and
+ * profiling).  No counterpart in the reference. */
+#define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone */
+#define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup */
+#define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
+int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated
  * string of at most cap bytes. */

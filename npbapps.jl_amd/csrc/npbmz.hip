@@ -31,18 +31,21 @@ __constant__ double kCE[5][13] = {
     {2.0, 2.0, 0.0, 0.0, 0.0, 2.0, 3.0, 0.03, 0.05, 0.04, 0.2, 0.1, 0.3},
     {5.0, 4.0, 3.0, 2.0, 0.1, 0.4, 0.3, 0.05, 0.04, 0.03, 0.1, 0.3, 0.2}};
 
-// A zone face copied by exch_qbc: the 5 components of every point of the
-// destination zone's boundary plane `type` (0 west i=0, 1 east i=nx-1,
-// 2 south j=0, 3 north j=ny-1) come from the source (neighbour) zone's
-// plane next to the shared boundary.
+// A zone face copied by exch_qbc: the 5 components of the destination zone's
+// boundary plane `type` (0 x-minus i=0, 1 x-plus i=nx-1, 2 y-minus j=0,
+// 3 y-plus j=ny-1) come from the source (neighbour) zone's plane next to the
+// shared boundary.  The copied points are the transverse index t0..t0+tn-1
+// (j for x faces, i for y faces) on planes k0, k0+1, ..., npts/tn of them.
 struct Face {
-  int32_t dst, src, type, npts;
+  int32_t dst, src, type, npts, t0, tn, k0, pad;
 };
 
-// exch_qbc for BT and SP (src/BT/exchange_qbc.jl:145-466, one rank): the
+// exch_qbc (src/BT/exchange_qbc.jl:145-466 for BT and SP, one rank): the
 // reference packs every face before unpacking any, and sources are interior
 // planes while destinations are boundary planes, so one kernel over all
-// faces reproduces it.  Copies are bit exact.
+// faces reproduces it.  LU's faces overlap at the zone corners and its
+// sources include the corners of other faces, so LU runs two of these
+// launches (see build_tables).  Copies are bit exact.
 template <int NT>
 __global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
                                                   const ZoneDev* __restrict__ zones,
@@ -55,13 +58,13 @@ __global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
   const ZoneDev D = zones[F.dst], S = zones[F.src];
   int64_t dp, sp;
   if (F.type < 2) {
-    const int j = 1 + idx % (D.ny - 2), k = 1 + idx / (D.ny - 2);
+    const int j = F.t0 + idx % F.tn, k = F.k0 + idx / F.tn;
     const int di = F.type == 0 ? 0 : D.nx - 1;
     const int si = F.type == 0 ? S.nx - 2 : 1;
     dp = D.base + ((int64_t)k * D.ny + j) * D.nx + di;
     sp = S.base + ((int64_t)k * S.ny + j) * S.nx + si;
   } else {
-    const int i = 1 + idx % (D.nx - 2), k = 1 + idx / (D.nx - 2);
+    const int i = F.t0 + idx % F.tn, k = F.k0 + idx / F.tn;
     const int dj = F.type == 2 ? 0 : D.ny - 1;
     const int sj = F.type == 2 ? S.ny - 2 : 1;
     dp = D.base + ((int64_t)k * D.ny + dj) * D.nx + i;
@@ -85,12 +88,14 @@ namespace exact {
 #pragma clang fp contract(off)
 #include "bt_kernels.inc"
 #include "sp_kernels.inc"
+#include "lu_kernels.inc"
 }  // namespace exact
 
 namespace fast {
 #pragma clang fp contract(fast)
 #include "bt_kernels.inc"
 #include "sp_kernels.inc"
+#include "lu_kernels.inc"
 }  // namespace fast
 
 }  // namespace npbmz
@@ -103,6 +108,11 @@ namespace {
 constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
+constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
+
+// LU sweep schedules: one launch per hyperplane over all zones, or one
+// workgroup per zone walking all hyperplanes of the sweep.
+enum { kSweepPlane = 0, kSweepZone = 1 };
 
 // A task table on the device plus, per zone, the range of its tasks.
 struct Table {
@@ -120,6 +130,7 @@ struct npbmz_ctx {
   std::vector<int> nx, ny, nz, west, east, south, north;
   std::vector<int> order;  // zone order of the batched tables
   StencilK K{};
+  LuK lk{};
   std::vector<ZoneDev> zh;
   ZoneDev* zd = nullptr;
   int64_t npoints = 0;
@@ -128,9 +139,12 @@ struct npbmz_ctx {
   size_t cscr_slot = 0;  // doubles per scratch slot (max over directions)
   int maxrows[3] = {0, 0, 0};
   int solve_grid[3] = {0, 0, 0};  // 0: one block per task, else persistent grid
-  Table t_int, t_all, t_line[3], t_face;
+  int lu_sweep = kSweepZone;      // LU sweep schedule
+  int lu_nt = 256;                // threads per workgroup of the per-zone sweep
+  Table t_int, t_all, t_line[3], t_face, t_face2;
   Face* faces = nullptr;
   double* partial = nullptr;
+  double* zpart = nullptr;  // one value per zone (LU surface integrals)
   std::vector<double> zn_rhs, zn_err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -209,6 +223,33 @@ StencilK make_constants(int bench, double dt, int gx, int gy, int gz, int x_zone
   return k;
 }
 
+// setcoeff (src/LU/setcoeff.jl:43-70) for nx0 x ny0 x nz0 zones, ssor's
+// relaxation factor (src/LU/ssor.jl:57) and jacld's scalars
+// (src/LU/jacld.jl:22-24).
+LuK make_lu_constants(double dt, double omega, int nx0, int ny0, int nz0) {
+  LuK k{};
+  k.dt = dt;
+  k.omega = omega;
+  k.tmp = 1.0 / (omega * (2.0 - omega));
+  k.dxi = 1.0 / (nx0 - 1);
+  k.deta = 1.0 / (ny0 - 1);
+  k.dzeta = 1.0 / (nz0 - 1);
+  k.tx1 = 1.0 / (k.dxi * k.dxi);
+  k.tx2 = 1.0 / (2.0 * k.dxi);
+  k.tx3 = 1.0 / k.dxi;
+  k.ty1 = 1.0 / (k.deta * k.deta);
+  k.ty2 = 1.0 / (2.0 * k.deta);
+  k.ty3 = 1.0 / k.deta;
+  k.tz1 = 1.0 / (k.dzeta * k.dzeta);
+  k.tz2 = 1.0 / (2.0 * k.dzeta);
+  k.tz3 = 1.0 / k.dzeta;
+  const double c1 = 1.40, c3 = 1.00e-01, c4 = 1.00, c5 = 1.40;
+  k.r43 = (4.0 / 3.0);
+  k.c1345 = c1 * c3 * c4 * c5;
+  k.c34 = c3 * c4;
+  return k;
+}
+
 int lines_of(const npbmz_ctx* c, int z, int dir) {
   return dir == 0 ? (c->ny[z] - 2) * (c->nz[z] - 2)
                   : dir == 1 ? (c->nx[z] - 2) * (c->nz[z] - 2) : (c->nx[z] - 2) * (c->ny[z] - 2);
@@ -219,9 +260,10 @@ int lines_of(const npbmz_ctx* c, int z, int dir) {
 template <class F>
 int build_table(npbmz_ctx* ctx, Table& t, const std::vector<int>& order, int per, F count) {
   std::vector<Task> h;
-  int nzn = (int)order.size();
-  t.zfirst.assign(nzn, 0);
-  t.zcount.assign(nzn, 0);
+  int nid = 0;
+  for (int z : order) nid = std::max(nid, z + 1);
+  t.zfirst.assign(nid, 0);
+  t.zcount.assign(nid, 0);
   for (int z : order) {
     int n = count(z);
     int nt = (n + per - 1) / per;
@@ -248,7 +290,7 @@ int build_tables(npbmz_ctx* ctx) {
                         [&](int z) { return c.nx[z] * c.ny[z] * c.nz[z]; })))
     return rc;
   size_t slot = 0;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < 3 && c.bench != NPBMZ_LU; ++d) {
     if ((rc = build_table(ctx, c.t_line[d], c.order, kLineNT,
                           [&](int z) { return lines_of(ctx, z, d); })))
       return rc;
@@ -272,29 +314,54 @@ int build_tables(npbmz_ctx* ctx) {
     CK(hipMalloc(&c.cscr, slot * sizeof(double)));
     c.cscr_slot = slot;
   }
-  // faces: west/east take (ny-2)(nz-2) points, south/north (nx-2)(nz-2)
+  // Faces.  BT and SP copy the interior of each face, j (or i) in 1..n-2 on
+  // planes k = 1..nz-2, all in one launch.
   std::vector<Face> fh;
+  std::vector<int> fa, fb;  // faces of the first and the second launch
   for (int z = 0; z < c.nz_; ++z) {
-    int nxf = (c.ny[z] - 2) * (c.nz[z] - 2), nyf = (c.nx[z] - 2) * (c.nz[z] - 2);
+    const int nx = c.nx[z], ny = c.ny[z], nz = c.nz[z];
+    if (c.bench == NPBMZ_LU) {
+      // LU (src/LU/exchange_qbc.jl:69-288) copies i faces over j = 0..ny-1
+      // and j faces over i = 0..nx-1, on planes k = 0..nz-3 as written or
+      // 1..nz-2 with NPBMZ_FLAG_LU_NPB_EXCH, and unpacks the j faces last, so
+      // they own the corners; every source plane is read before any face is
+      // written.  Launch 1 copies the j faces (their sources i = 0, nx-1 at
+      // j = 1, ny-2 are written by i faces); launch 2 copies the i faces
+      // without the corner rows (their sources j = 1..ny-2 are untouched by
+      // launch 1, and the corner rows would be overwritten anyway).
+      const int k0 = (c.flags & NPBMZ_FLAG_LU_NPB_EXCH) ? 1 : 0;
+      fa.push_back((int)fh.size());
+      fh.push_back(Face{z, c.south[z], 2, nx * (nz - 2), 0, nx, k0, 0});
+      fa.push_back((int)fh.size());
+      fh.push_back(Face{z, c.north[z], 3, nx * (nz - 2), 0, nx, k0, 0});
+      fb.push_back((int)fh.size());
+      fh.push_back(Face{z, c.west[z], 0, (ny - 2) * (nz - 2), 1, ny - 2, k0, 0});
+      fb.push_back((int)fh.size());
+      fh.push_back(Face{z, c.east[z], 1, (ny - 2) * (nz - 2), 1, ny - 2, k0, 0});
+      continue;
+    }
+    const int nxf = (ny - 2) * (nz - 2), nyf = (nx - 2) * (nz - 2);
     // SP's exchange fills j = 0 from the north neighbour and j = ny-1 from
     // the south one (src/SP/exchange_qbc.jl:247-295, 409-455), the mirror of
     // BT's (src/BT/exchange_qbc.jl:247-295, 409-455)
     const bool yswap = c.bench == NPBMZ_SP;
-    fh.push_back(Face{z, c.west[z], 0, nxf});
-    fh.push_back(Face{z, c.east[z], 1, nxf});
-    fh.push_back(Face{z, yswap ? c.north[z] : c.south[z], 2, nyf});
-    fh.push_back(Face{z, yswap ? c.south[z] : c.north[z], 3, nyf});
+    for (int q = 0; q < 4; ++q) fa.push_back((int)fh.size() + q);
+    fh.push_back(Face{z, c.west[z], 0, nxf, 1, ny - 2, 1, 0});
+    fh.push_back(Face{z, c.east[z], 1, nxf, 1, ny - 2, 1, 0});
+    fh.push_back(Face{z, yswap ? c.north[z] : c.south[z], 2, nyf, 1, nx - 2, 1, 0});
+    fh.push_back(Face{z, yswap ? c.south[z] : c.north[z], 3, nyf, 1, nx - 2, 1, 0});
   }
   if (!c.faces) CK(hipMalloc(&c.faces, sizeof(Face) * fh.size()));
   CK(hipMemcpy(c.faces, fh.data(), sizeof(Face) * fh.size(), hipMemcpyHostToDevice));
-  std::vector<int> forder(fh.size());
-  std::iota(forder.begin(), forder.end(), 0);
-  if ((rc = build_table(ctx, c.t_face, forder, kExchNT, [&](int f) { return fh[f].npts; })))
+  if ((rc = build_table(ctx, c.t_face, fa, kExchNT, [&](int f) { return fh[f].npts; })))
+    return rc;
+  if ((rc = build_table(ctx, c.t_face2, fb, kExchNT, [&](int f) { return fh[f].npts; })))
     return rc;
   int pmax = std::max(c.t_int.n, c.t_all.n);
   if (c.partial) (void)hipFree(c.partial);
   c.partial = nullptr;
-  CK(hipMalloc(&c.partial, sizeof(double) * 5 * (size_t)std::max(pmax, 1)));
+  CK(hipMalloc(&c.partial, sizeof(double) * 10 * (size_t)std::max(pmax, 1)));
+  if (!c.zpart) CK(hipMalloc(&c.zpart, sizeof(double) * (size_t)c.nz_));
   if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
   if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
   return NPBMZ_OK;
@@ -373,12 +440,112 @@ int k_solve(npbmz_ctx* ctx, int dir, bool fuse, int zone) {
 
 int exch(npbmz_ctx* ctx) {
   auto& c = *ctx;
-  if (c.nz_ <= 1) return NPBMZ_OK;  // bt-node.jl:286: exchange only with >1 zone
-  if (c.t_face.n <= 0) return NPBMZ_OK;
-  hipLaunchKernelGGL(exch_kernel<kExchNT>, dim3(c.t_face.n), dim3(kExchNT), 0, c.stream, c.u,
-                     c.zd, c.faces, c.t_face.d);
+  // bt-node.jl:286 and sp-node.jl:258 exchange only with more than one zone;
+  // lu-node.jl:344 always does
+  if (c.nz_ <= 1 && c.bench != NPBMZ_LU) return NPBMZ_OK;
+  const Table* ts[2] = {&c.t_face, &c.t_face2};
+  for (const Table* t : ts) {
+    if (t->n <= 0) continue;
+    hipLaunchKernelGGL(exch_kernel<kExchNT>, dim3(t->n), dim3(kExchNT), 0, c.stream, c.u, c.zd,
+                       c.faces, t->d);
+    CK(hipGetLastError());
+  }
+  return NPBMZ_OK;
+}
+
+// ------------------------------------------------------------------- LU
+// rhs (erhs = false) or erhs (erhs = true, src = the staged polynomial in
+// rsd, out = frct) over every point of one zone (zone >= 0) or all zones.
+int k_lu_rhs(npbmz_ctx* ctx, int zone, bool erhs = false) {
+  auto& c = *ctx;
+  const Table& t = c.t_all;
+  int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
+  if (n <= 0) return NPBMZ_OK;
+  auto go = [&](auto kern, const double* src, double* out) {
+    hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, src, c.frc, out, c.zd, t.d + first,
+                       c.lk);
+  };
+  if (erhs) go(exact::lu::rhs_kernel<kPtNT, true>, c.rhs, c.frc);  // setup: always exact
+  else if (strict(ctx)) go(exact::lu::rhs_kernel<kPtNT, false>, c.u, c.rhs);
+  else go(fast::lu::rhs_kernel<kPtNT, false>, c.u, c.rhs);
   CK(hipGetLastError());
   return NPBMZ_OK;
+}
+
+int k_lu_update(npbmz_ctx* ctx, int zone) {
+  auto& c = *ctx;
+  const Table& t = c.t_int;
+  int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
+  if (n <= 0) return NPBMZ_OK;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, c.u, c.rhs, c.zd, t.d + first, c.lk);
+  };
+  if (strict(ctx)) go(exact::lu::update_kernel<kPtNT>);
+  else go(fast::lu::update_kernel<kPtNT>);
+  CK(hipGetLastError());
+  return NPBMZ_OK;
+}
+
+// One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
+// zone (zone >= 0) or all zones, in the context's schedule.
+template <bool UPPER>
+int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
+  auto& c = *ctx;
+  const int z0 = zone < 0 ? 0 : zone, nzl = zone < 0 ? c.nz_ : 1;
+  const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
+  const int ns = NI + NJ + NK - 2;
+  if (c.lu_sweep == kSweepZone) {
+    auto go = [&](auto kern, int nt) {
+      hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), 0, c.stream, c.u, c.rhs, c.zd, z0, c.lk);
+    };
+    const bool s = strict(ctx);
+    switch (c.lu_nt) {
+      case 128: s ? go(exact::lu::sweep_zone_kernel<UPPER, 128>, 128) : go(fast::lu::sweep_zone_kernel<UPPER, 128>, 128); break;
+      case 512: s ? go(exact::lu::sweep_zone_kernel<UPPER, 512>, 512) : go(fast::lu::sweep_zone_kernel<UPPER, 512>, 512); break;
+      default: s ? go(exact::lu::sweep_zone_kernel<UPPER, 256>, 256) : go(fast::lu::sweep_zone_kernel<UPPER, 256>, 256); break;
+    }
+    CK(hipGetLastError());
+    return NPBMZ_OK;
+  }
+  for (int q = 0; q < ns; ++q) {
+    const int s = UPPER ? ns - 1 - q : q;
+    const int c_lo = std::max(0, s - (NI - 1) - (NJ - 1)), c_hi = std::min(NK - 1, s);
+    const int npair = (c_hi - c_lo + 1) * NJ;
+    const dim3 grid((npair + kPlaneNT - 1) / kPlaneNT, nzl);
+    if (strict(ctx))
+      hipLaunchKernelGGL((exact::lu::sweep_plane_kernel<UPPER, kPlaneNT>), grid, dim3(kPlaneNT), 0,
+                         c.stream, c.u, c.rhs, c.zd, z0, s, c.lk);
+    else
+      hipLaunchKernelGGL((fast::lu::sweep_plane_kernel<UPPER, kPlaneNT>), grid, dim3(kPlaneNT), 0,
+                         c.stream, c.u, c.rhs, c.zd, z0, s, c.lk);
+    CK(hipGetLastError());
+  }
+  return NPBMZ_OK;
+}
+
+int k_lu_sweep(npbmz_ctx* ctx, bool upper, int zone) {
+  return upper ? k_lu_sweep_t<true>(ctx, zone) : k_lu_sweep_t<false>(ctx, zone);
+}
+
+// One LU iteration (src/LU/lu-node.jl:338-428): exch_qbc, then per zone
+// ssor (src/LU/ssor.jl:9-254): lower sweep with the dt scaling folded in,
+// upper sweep, update, rhs.
+int ssor_step(npbmz_ctx* ctx, int mode) {
+  int rc;
+  if ((rc = exch(ctx))) return rc;
+  if (mode == NPBMZ_MODE_PERZONE) {
+    for (int z = 0; z < ctx->nz_; ++z) {
+      if ((rc = k_lu_sweep(ctx, false, z))) return rc;
+      if ((rc = k_lu_sweep(ctx, true, z))) return rc;
+      if ((rc = k_lu_update(ctx, z))) return rc;
+      if ((rc = k_lu_rhs(ctx, z))) return rc;
+    }
+    return NPBMZ_OK;
+  }
+  if ((rc = k_lu_sweep(ctx, false, -1))) return rc;
+  if ((rc = k_lu_sweep(ctx, true, -1))) return rc;
+  if ((rc = k_lu_update(ctx, -1))) return rc;
+  return k_lu_rhs(ctx, -1);
 }
 
 // One BT timestep: exch_qbc, then per zone compute_rhs, x/y/z solve, add.
@@ -406,12 +573,27 @@ int adi_step(npbmz_ctx* ctx, int mode) {
   return NPBMZ_OK;
 }
 
+int step_once(npbmz_ctx* ctx, int mode) {
+  return ctx->bench == NPBMZ_LU ? ssor_step(ctx, mode) : adi_step(ctx, mode);
+}
+
 int run_phase(npbmz_ctx* ctx, int phase, int mode) {
   int zlo = 0, zhi = ctx->nz_;
   if (mode != NPBMZ_MODE_PERZONE) { zlo = -1; zhi = 0; }
   int rc = NPBMZ_OK;
   if (phase == NPBMZ_PHASE_EXCH) return exch(ctx);
   for (int z = zlo; z < zhi; ++z) {
+    if (ctx->bench == NPBMZ_LU) {
+      switch (phase) {
+        case NPBMZ_PHASE_RHS: rc = k_lu_rhs(ctx, z); break;
+        case NPBMZ_PHASE_SSOR_LOWER: rc = k_lu_sweep(ctx, false, z); break;
+        case NPBMZ_PHASE_SSOR_UPPER: rc = k_lu_sweep(ctx, true, z); break;
+        case NPBMZ_PHASE_SSOR_UPDATE: rc = k_lu_update(ctx, z); break;
+        default: return fail(ctx, NPBMZ_EINVAL, "unknown LU phase");
+      }
+      if (rc) return rc;
+      continue;
+    }
     switch (phase) {
       case NPBMZ_PHASE_RHS: rc = k_rhs(ctx, z); break;
       case NPBMZ_PHASE_XSOLVE: rc = k_solve(ctx, 0, false, z); break;
@@ -429,7 +611,7 @@ int ensure_graph(npbmz_ctx* ctx) {
   auto& c = *ctx;
   if (c.gexec) return NPBMZ_OK;
   CK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  int rc = adi_step(ctx, NPBMZ_MODE_BATCHED);
+  int rc = step_once(ctx, NPBMZ_MODE_BATCHED);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c.stream, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -441,9 +623,14 @@ int ensure_graph(npbmz_ctx* ctx) {
 
 void describe(npbmz_ctx* c) {
   char b[256];
-  snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d fp=%s", c->nz_,
-           c->order.empty() || c->order[0] == 0 ? "natural" : "largest-first", c->solve_grid[0],
-           c->solve_grid[1], c->solve_grid[2], strict(c) ? "strict" : "fma");
+  if (c->bench == NPBMZ_LU)
+    snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d exch=%s fp=%s", c->nz_,
+             c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
+             (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
+  else
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d fp=%s", c->nz_,
+             c->order.empty() || c->order[0] == 0 ? "natural" : "largest-first", c->solve_grid[0],
+             c->solve_grid[1], c->solve_grid[2], strict(c) ? "strict" : "fma");
   c->sched = b;
 }
 
@@ -459,8 +646,8 @@ const char* npbmz_last_error(const npbmz_ctx* ctx) {
 int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   if (!p || !out) return fail(nullptr, NPBMZ_EINVAL, "null argument");
   *out = nullptr;
-  if (p->bench != NPBMZ_BT && p->bench != NPBMZ_SP)
-    return fail(nullptr, NPBMZ_ENOTSUP, "bench must be NPBMZ_BT or NPBMZ_SP");
+  if (p->bench != NPBMZ_BT && p->bench != NPBMZ_SP && p->bench != NPBMZ_LU)
+    return fail(nullptr, NPBMZ_ENOTSUP, "bench must be NPBMZ_BT, NPBMZ_SP or NPBMZ_LU");
   if (p->nzones <= 0 || p->nzones != p->x_zones * p->y_zones)
     return fail(nullptr, NPBMZ_EINVAL, "nzones must equal x_zones*y_zones and be positive");
   if (!p->nx || !p->ny || !p->nz || !p->iz_west || !p->iz_east || !p->iz_south || !p->iz_north)
@@ -469,6 +656,15 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   if (p->gx_size < 2 || p->gy_size < 2 || p->gz_size < 2)
     return fail(nullptr, NPBMZ_EINVAL, "grid sizes must be at least 2");
   int n = p->nzones;
+  if (p->bench == NPBMZ_LU) {
+    if (!(p->omega > 0.0 && p->omega < 2.0))
+      return fail(nullptr, NPBMZ_EINVAL, "LU needs 0 < omega < 2");
+    // src/LU/zone_setup.jl:22-33 splits the grid into equal zones, and the
+    // node's setcoeff globals (src/LU/setcoeff.jl:43-70) assume it
+    for (int z = 1; z < n; ++z)
+      if (p->nx[z] != p->nx[0] || p->ny[z] != p->ny[0] || p->nz[z] != p->nz[0])
+        return fail(nullptr, NPBMZ_EINVAL, "LU zones must all have the same size");
+  }
   for (int z = 0; z < n; ++z) {
     // below 5 points the reference's dissipation stencils read its halo
     // values (rhs.jl:158-210, lhsx.jl:48-90), which this library does not keep
@@ -509,6 +705,7 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   c.west.assign(p->iz_west, p->iz_west + n); c.east.assign(p->iz_east, p->iz_east + n);
   c.south.assign(p->iz_south, p->iz_south + n); c.north.assign(p->iz_north, p->iz_north + n);
   c.K = make_constants(c.bench, c.dt, c.gx, c.gy, c.gz, c.x_zones, c.y_zones);
+  if (c.bench == NPBMZ_LU) c.lk = make_lu_constants(c.dt, c.omega, c.nx[0], c.ny[0], c.nz[0]);
   c.zh.resize(n);
   int64_t base = 0;
   for (int z = 0; z < n; ++z) {
@@ -561,10 +758,11 @@ void npbmz_destroy(npbmz_ctx* ctx) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   if (c.gexec) (void)hipGraphExecDestroy(c.gexec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
-  Table* ts[] = {&c.t_int, &c.t_all, &c.t_line[0], &c.t_line[1], &c.t_line[2], &c.t_face};
+  Table* ts[] = {&c.t_int,     &c.t_all,  &c.t_line[0], &c.t_line[1],
+                 &c.t_line[2], &c.t_face, &c.t_face2};
   for (Table* t : ts)
     if (t->d) (void)hipFree(t->d);
-  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial};
+  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -661,6 +859,21 @@ int npbmz_init_exact(npbmz_ctx* ctx) {
   if (t.n <= 0) return NPBMZ_OK;
   // setup arithmetic always runs without contraction: it is the reference's
   // initial state, not part of the timed step
+  if (c.bench == NPBMZ_LU) {
+    // setbv, setiv, erhs and the first rhs (src/LU/lu-node.jl:145-196); the
+    // warm-up iteration and re-initialisation that follow (:203-311) end in
+    // exactly this state again
+    hipLaunchKernelGGL(exact::lu::init_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, c.u,
+                       c.rhs, c.frc, c.zd, t.d);
+    CK(hipGetLastError());
+    int rc;
+    if ((rc = k_lu_rhs(ctx, -1, true))) return rc;
+    hipLaunchKernelGGL((exact::lu::rhs_kernel<kPtNT, false>), dim3(t.n), dim3(kPtNT), 0, c.stream,
+                       c.u, c.frc, c.rhs, c.zd, t.d, c.lk);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(c.stream));
+    return NPBMZ_OK;
+  }
   hipLaunchKernelGGL(exact::bt::init_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, c.u,
                      c.zd, t.d, c.K);
   CK(hipGetLastError());
@@ -684,7 +897,7 @@ int npbmz_step(npbmz_ctx* ctx, int32_t nsteps, int32_t mode, double* elapsed_ms)
   for (int s = 0; s < nsteps; ++s) {
     if (mode == NPBMZ_MODE_GRAPH) {
       CK(hipGraphLaunch(c.gexec, c.stream));
-    } else if ((rc = adi_step(ctx, mode))) {
+    } else if ((rc = step_once(ctx, mode))) {
       return rc;
     }
   }
@@ -710,7 +923,9 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
   if (!ctx) return NPBMZ_EINVAL;
   if (nsteps <= 0 || !phase_ms) return fail(ctx, NPBMZ_EINVAL, "nsteps <= 0 or null output");
   auto& c = *ctx;
-  const int nph = 5;  // exch, rhs, x, y, z(+add)
+  const bool lu = c.bench == NPBMZ_LU;
+  // BT/SP: exch, rhs, x, y, z(+add); LU: exch, lower, upper, update, rhs
+  const int nph = 5;
   std::vector<hipEvent_t> ev((size_t)nsteps * nph * 2, nullptr);
   int rc = NPBMZ_OK;
   for (auto& e : ev)
@@ -720,6 +935,9 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
       hipEvent_t* e = &ev[((size_t)s * nph + ph) * 2];
       if (hipEventRecord(e[0], c.stream) != hipSuccess) { rc = fail(ctx, NPBMZ_EHIP, "record"); break; }
       if (ph == 0) rc = exch(ctx);
+      else if (lu) rc = ph == 1 ? k_lu_sweep(ctx, false, -1)
+                       : ph == 2 ? k_lu_sweep(ctx, true, -1)
+                       : ph == 3 ? k_lu_update(ctx, -1) : k_lu_rhs(ctx, -1);
       else if (ph == 1) rc = k_rhs(ctx, -1);
       else rc = k_solve(ctx, ph - 2, true, -1);
       if (rc == NPBMZ_OK && hipEventRecord(e[1], c.stream) != hipSuccess)
@@ -738,7 +956,10 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
           rc = fail(ctx, NPBMZ_EHIP, "hipEventElapsedTime");
           break;
         }
-        phase_ms[ph] += ms;
+        static const int lu_slot[5] = {NPBMZ_PHASE_EXCH, NPBMZ_PHASE_SSOR_LOWER,
+                                       NPBMZ_PHASE_SSOR_UPPER, NPBMZ_PHASE_SSOR_UPDATE,
+                                       NPBMZ_PHASE_RHS};
+        phase_ms[lu ? lu_slot[ph] : ph] += ms;
       }
   }
   for (auto& e : ev)
@@ -750,8 +971,44 @@ int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci) {
   if (!ctx) return NPBMZ_EINVAL;
   if (!xcr || !xce) return fail(ctx, NPBMZ_EINVAL, "null output");
   auto& c = *ctx;
-  (void)xci;
   const int nzn = c.nz_;
+  if (c.bench == NPBMZ_LU) {
+    // src/LU/lu-node.jl:433-458: per zone ERROR, l2norm of the current rsd
+    // (the last ssor ended with rhs) and pintgr, summed in zone order
+    const Table& t = c.t_int;
+    std::vector<double> pp((size_t)t.n * 10), fz((size_t)nzn);
+    hipLaunchKernelGGL(exact::lu::norm_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, c.u,
+                       c.rhs, c.zd, t.d, c.partial);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(exact::lu::pintgr_kernel<256>, dim3(nzn), dim3(256), 0, c.stream, c.u, c.zd,
+                       c.lk, c.zpart);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(pp.data(), c.partial, pp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      c.stream));
+    CK(hipMemcpyAsync(fz.data(), c.zpart, fz.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      c.stream));
+    CK(hipStreamSynchronize(c.stream));
+    c.zn_rhs.assign((size_t)nzn * 5, 0.0);
+    c.zn_err.assign((size_t)nzn * 5, 0.0);
+    for (int m = 0; m < 5; ++m) { xcr[m] = 0.0; xce[m] = 0.0; }
+    double frc = 0.0;
+    for (int z = 0; z < nzn; ++z) {
+      double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int b = 0; b < t.zcount[z]; ++b)
+        for (int q = 0; q < 10; ++q) s[q] += pp[(size_t)(t.zfirst[z] + b) * 10 + q];
+      const double vol = double(c.nx[z] - 2) * (c.ny[z] - 2) * (c.nz[z] - 2);
+      for (int m = 0; m < 5; ++m) {
+        c.zn_err[(size_t)z * 5 + m] = std::sqrt(s[m] / vol);
+        c.zn_rhs[(size_t)z * 5 + m] = std::sqrt(s[5 + m] / vol);
+        xce[m] = xce[m] + c.zn_err[(size_t)z * 5 + m];
+        xcr[m] = xcr[m] + c.zn_rhs[(size_t)z * 5 + m];
+      }
+      frc = frc + fz[(size_t)z];
+    }
+    if (xci) *xci = frc;
+    return NPBMZ_OK;
+  }
+  if (xci) *xci = 0.0;
   std::vector<double> pe((size_t)c.t_all.n * 5), pr((size_t)c.t_int.n * 5);
   // error_norm on every zone's current u
   if (strict(ctx))
@@ -806,7 +1063,101 @@ int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err) {
 
 int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   if (!ctx) return NPBMZ_EINVAL;
-  (void)budget_ms;
+  if (budget_ms < 0) return fail(ctx, NPBMZ_EINVAL, "budget_ms < 0");
+  auto& c = *ctx;
+  // Candidate schedules.  LU: hyperplane launches, or one workgroup per zone
+  // of 128/256/512 threads.  BT/SP: zone order of the task tables x solve
+  // grid (one block per task, or a persistent grid that strides over them).
+  struct Cand { int a, b; };
+  std::vector<Cand> cands;
+  if (c.bench == NPBMZ_LU) {
+    cands = {{kSweepPlane, 0}, {kSweepZone, 128}, {kSweepZone, 256}, {kSweepZone, 512}};
+  } else {
+    for (int ord = 0; ord < 2; ++ord)
+      for (int g : {0, 1024, 2048}) cands.push_back({ord, g});
+  }
+  auto apply = [&](const Cand& k) -> int {
+    if (c.bench == NPBMZ_LU) {
+      c.lu_sweep = k.a;
+      c.lu_nt = k.b ? k.b : c.lu_nt;
+    } else {
+      std::iota(c.order.begin(), c.order.end(), 0);
+      if (k.a == 0)
+        std::stable_sort(c.order.begin(), c.order.end(), [&](int x, int y) {
+          return (int64_t)c.nx[x] * c.ny[x] * c.nz[x] > (int64_t)c.nx[y] * c.ny[y] * c.nz[y];
+        });
+      for (int d = 0; d < 3; ++d) c.solve_grid[d] = k.b;
+      int rc = build_tables(ctx);
+      if (rc) return rc;
+    }
+    if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
+    if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+    return NPBMZ_OK;
+  };
+  // the state is saved and restored around the trial steps
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  double *su = nullptr, *sr = nullptr;
+  CK(hipMalloc(&su, fb));
+  if (hipMalloc(&sr, fb) != hipSuccess) { (void)hipFree(su); return fail(ctx, NPBMZ_ENOMEM, "tune buffers"); }
+  int rc = NPBMZ_OK;
+  if (hipMemcpyAsync(su, c.u, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+      hipMemcpyAsync(sr, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+    rc = fail(ctx, NPBMZ_EHIP, "tune save");
+  const double per = std::max(1.0, double(budget_ms) / cands.size());
+  double best = 1e300;
+  size_t bi = 0;
+  for (size_t q = 0; q < cands.size() && rc == NPBMZ_OK; ++q) {
+    if ((rc = apply(cands[q]))) break;
+    double ms = 0.0, tot = 0.0;
+    int steps = 0;
+    if ((rc = npbmz_step(ctx, 1, NPBMZ_MODE_BATCHED, &ms))) break;  // warm-up
+    tot = 0.0;
+    do {
+      if ((rc = npbmz_step(ctx, 1, NPBMZ_MODE_BATCHED, &ms))) break;
+      tot += ms;
+      ++steps;
+    } while (tot < per && steps < 50);
+    if (rc) break;
+    const double t = tot / steps;
+    if (t < best) { best = t; bi = q; }
+    if (hipMemcpyAsync(c.u, su, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+        hipMemcpyAsync(c.rhs, sr, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+      rc = fail(ctx, NPBMZ_EHIP, "tune restore");
+  }
+  if (rc == NPBMZ_OK) rc = apply(cands[bi]);
+  if (hipStreamSynchronize(c.stream) != hipSuccess && rc == NPBMZ_OK)
+    rc = fail(ctx, NPBMZ_EHIP, "tune sync");
+  (void)hipFree(su);
+  (void)hipFree(sr);
+  describe(ctx);
+  return rc;
+}
+
+int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
+  if (!ctx) return NPBMZ_EINVAL;
+  auto& c = *ctx;
+  switch (which) {
+    case NPBMZ_SCHED_LU_SWEEP:
+      if (c.bench != NPBMZ_LU || (value != 0 && value != 1))
+        return fail(ctx, NPBMZ_EINVAL, "LU sweep schedule is 0 (hyperplane) or 1 (zone)");
+      c.lu_sweep = value;
+      break;
+    case NPBMZ_SCHED_LU_THREADS:
+      if (c.bench != NPBMZ_LU || (value != 128 && value != 256 && value != 512))
+        return fail(ctx, NPBMZ_EINVAL, "LU workgroup size is 128, 256 or 512");
+      c.lu_nt = value;
+      break;
+    case NPBMZ_SCHED_SOLVE_GRID: {
+      if (c.bench == NPBMZ_LU || value < 0) return fail(ctx, NPBMZ_EINVAL, "bad solve grid");
+      for (int d = 0; d < 3; ++d) c.solve_grid[d] = value;
+      int rc = build_tables(ctx);
+      if (rc) return rc;
+      break;
+    }
+    default: return fail(ctx, NPBMZ_EINVAL, "unknown schedule knob");
+  }
+  if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
+  if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
   describe(ctx);
   return NPBMZ_OK;
 }

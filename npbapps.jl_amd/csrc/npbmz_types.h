@@ -44,4 +44,15 @@ struct StencilK {
   double dmax[3];     // dxmax, dymax, dzmax
 };
 
+// LU-MZ constants: setcoeff (src/LU/setcoeff.jl:43-70) for the zone size,
+// omega (src/LU/lu_data.jl:8) and the jacld/jacu scalars r43, c34, c1345
+// (src/LU/jacld.jl:22-24).  LU zones all have the same size, so one set
+// serves every zone.
+struct LuK {
+  double dt, omega, tmp;  // tmp = 1/(omega*(2-omega)), src/LU/ssor.jl:57
+  double tx1, tx2, tx3, ty1, ty2, ty3, tz1, tz2, tz3;
+  double dxi, deta, dzeta;
+  double r43, c34, c1345;
+};
+
 }  // namespace npbmz

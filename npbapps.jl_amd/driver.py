@@ -12,10 +12,12 @@ from .zones import CLASSES, mflops_per_step, zones_for
 
 
 def go(bench="BT", cls="S", mode="graph", niter=None, strict=False, warmup=True,
-       verbose=True, tune_ms=0):
+       verbose=True, tune_ms=0, npb_exchange=False):
     """Run one benchmark class end to end on the current GPU.
 
-    bench: "BT" (this library also builds "SP" and "LU" where enabled).
+    bench: "BT", "SP" or "LU".  npb_exchange (LU): exchange planes
+    k = 2..nz-1 as NPB-MZ does; the reference's own planes k = 1..nz-2
+    (the default) do not reproduce its verification tables.
     mode: "perzone" (one launch per zone per phase), "batched" (one launch per
     phase over all zones) or "graph" (batched timestep replayed from a
     hipGraph).  strict: use the kernels built without FMA contraction
@@ -25,13 +27,15 @@ def go(bench="BT", cls="S", mode="graph", niter=None, strict=False, warmup=True,
     zones, p = zones_for(bench, cls)
     niter = p.niter if niter is None else int(niter)
     t0 = time.time()
-    ctx = native.Context(bench, zones, p.dt, omega=1.2, strict=strict)
+    # omega: src/LU/lu_data.jl:8 (BT and SP ignore it)
+    ctx = native.Context(bench, zones, p.dt, omega=1.2, strict=strict, npb_exchange=npb_exchange)
     try:
         ctx.init_exact()
         if tune_ms:
             ctx.tune(tune_ms)
         if warmup:
-            # bt-node.jl:138-253: one untimed step, then u is re-initialised
+            # bt-node.jl:138-253 (lu-node.jl:203-311): one untimed step, then
+            # the state is re-initialised
             ctx.step(1, mode)
             ctx.init_exact()
         setup_s = time.time() - t0
@@ -47,7 +51,7 @@ def go(bench="BT", cls="S", mode="graph", niter=None, strict=False, warmup=True,
         "grid": (p.gx, p.gy, p.gz), "zones": (p.x_zones, p.y_zones), "npoints": zones.npoints,
         "time_s": t, "mops": mflops, "gpts_per_s": zones.npoints * niter / t if t > 0 else 0.0,
         "xcr": list(xcr), "xce": list(xce), "xci": xci, "verified": ok, "rel": rel,
-        "setup_s": setup_s, "schedule": sched, "strict": strict,
+        "setup_s": setup_s, "schedule": sched, "strict": strict, "npb_exchange": npb_exchange,
     }
     if verbose:
         print_results(res)
@@ -70,3 +74,5 @@ def print_results(r):
         for m in range(5):
             print(f"   {m + 1} xcr {r['xcr'][m]:20.13E} rel {r['rel']['xcr'][m]:.2E}   "
                   f"xce {r['xce'][m]:20.13E} rel {r['rel']['xce'][m]:.2E}")
+        if r["rel"].get("xci") is not None:
+            print(f"     xci {r['xci']:20.13E} rel {r['rel']['xci']:.2E}")

@@ -16,18 +16,22 @@ BT, SP, LU = 0, 1, 2
 BENCH = {"BT": BT, "SP": SP, "LU": LU}
 
 FLAG_STRICT_FP = 1
+FLAG_LU_NPB_EXCH = 2
 FIELD_U, FIELD_FORCING, FIELD_RHS = 0, 1, 2
 ORDER_MIJK, ORDER_IJKM = 0, 1
 MODE_PERZONE, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
 MODES = {"perzone": MODE_PERZONE, "batched": MODE_BATCHED, "graph": MODE_GRAPH}
 PHASE_EXCH, PHASE_RHS, PHASE_XSOLVE, PHASE_YSOLVE, PHASE_ZSOLVE, PHASE_ADD = range(6)
+# LU's phases (exch and rhs as above)
+PHASE_SSOR_LOWER, PHASE_SSOR_UPPER, PHASE_SSOR_UPDATE = 2, 3, 4
+SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID = 0, 1, 2
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (
     "npbmz_create", "npbmz_destroy", "npbmz_last_error", "npbmz_put_field",
     "npbmz_get_field", "npbmz_init_exact", "npbmz_step", "npbmz_run_phase", "npbmz_phase_times",
     "npbmz_norms", "npbmz_zone_norms", "npbmz_tune", "npbmz_describe", "npbmz_size",
-    "npbmz_probe_copy",
+    "npbmz_probe_copy", "npbmz_set_schedule",
 )
 
 
@@ -80,6 +84,7 @@ def load(path=LIB_PATH):
     lib.npbmz_describe.argtypes = [P, ctypes.c_char_p, i32]
     lib.npbmz_size.argtypes = [P, ctypes.POINTER(i32), i64p]
     lib.npbmz_probe_copy.argtypes = [P, i64, i32, dp]
+    lib.npbmz_set_schedule.argtypes = [P, i32, i32]
     for f in EXPORTS:
         if f not in ("npbmz_destroy", "npbmz_last_error"):
             getattr(lib, f).restype = ctypes.c_int
@@ -98,7 +103,11 @@ def _i32(a):
 class Context:
     """One libnpbmz context: a multi-zone problem resident in HBM."""
 
-    def __init__(self, bench, zones, dt, omega=0.0, strict=False):
+    def __init__(self, bench, zones, dt, omega=0.0, strict=False, npb_exchange=False):
+        """bench "BT", "SP" or "LU"; zones from zones.zone_setup; strict selects
+        the kernels built without FMA contraction; npb_exchange (LU only)
+        exchanges planes k = 2..nz-1 as NPB-MZ does instead of the
+        reference's k = 1..nz-2."""
         lib = load()
         self._lib = lib
         self.bench = BENCH[bench] if isinstance(bench, str) else int(bench)
@@ -110,7 +119,8 @@ class Context:
         p = Problem(self.bench, n, zones.x_zones, zones.y_zones, zones.gx, zones.gy, zones.gz,
                     *self._keep, float(dt), float(omega))
         h = ctypes.c_void_p()
-        rc = lib.npbmz_create(ctypes.byref(p), FLAG_STRICT_FP if strict else 0, ctypes.byref(h))
+        flags = (FLAG_STRICT_FP if strict else 0) | (FLAG_LU_NPB_EXCH if npb_exchange else 0)
+        rc = lib.npbmz_create(ctypes.byref(p), flags, ctypes.byref(h))
         if rc != 0:
             raise NpbmzError(f"npbmz_create -> {rc}: {lib.npbmz_last_error(None).decode()}")
         self._h = h
@@ -195,8 +205,9 @@ class Context:
         self._check(self._lib.npbmz_run_phase(self._h, int(phase), m), "npbmz_run_phase")
 
     def phase_times(self, nsteps):
-        """Seconds spent in each phase (exch, rhs, x, y, z[+add], add) summed
-        over nsteps batched timesteps, from HIP events on the context's stream."""
+        """Seconds spent in each phase summed over nsteps batched timesteps,
+        from HIP events on the context's stream, indexed by PHASE_*: BT/SP
+        exch, rhs, x, y, z(+add), add; LU exch, rhs, lower, upper, update."""
         ms = np.zeros(6)
         self._check(self._lib.npbmz_phase_times(self._h, int(nsteps), _dptr(ms)),
                     "npbmz_phase_times")
@@ -219,6 +230,10 @@ class Context:
 
     def tune(self, budget_ms=2000):
         self._check(self._lib.npbmz_tune(self._h, int(budget_ms)), "npbmz_tune")
+
+    def set_schedule(self, which, value):
+        self._check(self._lib.npbmz_set_schedule(self._h, int(which), int(value)),
+                    "npbmz_set_schedule")
 
     def probe_copy(self, nbytes, reps=20):
         """Seconds per launch of a plain nbytes device-to-device copy kernel."""

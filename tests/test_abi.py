@@ -49,6 +49,26 @@ def test_create_validates_before_touching_a_device():
     del ctx
 
 
+def test_create_validates_lu_problems():
+    """LU needs 0 < omega < 2 and equal zones (src/LU/zone_setup.jl:22-33)."""
+    lib = native.load()
+    z, p = P.zones_for("LU", "S")
+    n = z.count
+    arrs = [list(v) for v in (z.nx, z.ny, z.nz, z.west, z.east, z.south, z.north)]
+
+    def create(arrs, omega):
+        c = [(ctypes.c_int32 * n)(*v) for v in arrs]
+        q = native.Problem(native.LU, n, 4, 4, 24, 24, 6, *c, 0.5, omega)
+        h = ctypes.c_void_p()
+        return lib.npbmz_create(ctypes.byref(q), 0, ctypes.byref(h))
+
+    assert create(arrs, 2.0) == -1 and b"omega" in lib.npbmz_last_error(None)
+    odd = [list(v) for v in arrs]
+    odd[2][3] += 1
+    assert create(odd, 1.2) == -1
+    assert b"same size" in lib.npbmz_last_error(None) or b"neighbours" in lib.npbmz_last_error(None)
+
+
 def test_missing_library_fails_loudly(tmp_path):
     native_mod = native
     with pytest.raises(native_mod.NpbmzError, match="not built"):
