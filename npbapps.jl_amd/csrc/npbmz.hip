@@ -94,6 +94,7 @@ namespace exact {
 namespace fast {
 #pragma clang fp contract(fast)
 #include "bt_kernels.inc"
+#include "bt_adjoint.inc"
 #include "sp_kernels.inc"
 #include "lu_kernels.inc"
 }  // namespace fast
@@ -570,6 +571,83 @@ int adi_step(npbmz_ctx* ctx, int mode) {
   if ((rc = k_solve(ctx, 0, false, -1))) return rc;
   if ((rc = k_solve(ctx, 1, false, -1))) return rc;
   if ((rc = k_solve(ctx, 2, true, -1))) return rc;
+  return NPBMZ_OK;
+}
+
+// ------------------------------------------------------------ BT adjoint
+// One batched BT step with the z solve and add as separate launches, copying
+// the post-exchange u and the outputs of the three solves when the targets
+// are given (the forward pass and the segment recomputation of the adjoint
+// run the same launches, so they follow the same trajectory).
+int bt_step_store(npbmz_ctx* ctx, double* us, double* x1, double* x2, double* x3) {
+  auto& c = *ctx;
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  int rc;
+  if ((rc = exch(ctx))) return rc;
+  if (us) CK(hipMemcpyAsync(us, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_rhs(ctx, -1))) return rc;
+  if ((rc = k_solve(ctx, 0, false, -1))) return rc;
+  if (x1) CK(hipMemcpyAsync(x1, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_solve(ctx, 1, false, -1))) return rc;
+  if (x2) CK(hipMemcpyAsync(x2, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_solve(ctx, 2, false, -1))) return rc;
+  if (x3) CK(hipMemcpyAsync(x3, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  return k_add(ctx, -1);
+}
+
+template <int DIR>
+int k_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double* gin, double* lam,
+                double* ubar) {
+  auto& c = *ctx;
+  const Table& t = c.t_line[DIR];
+  if (t.n <= 0) return NPBMZ_OK;
+  int grid = t.n;
+  if (c.solve_grid[DIR] > 0) grid = std::min(t.n, c.solve_grid[DIR]);
+  hipLaunchKernelGGL(fast::bt::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us, x,
+                     gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  CK(hipGetLastError());
+  return NPBMZ_OK;
+}
+
+// Reverse one BT step from its stored forward data (us: u after the
+// exchange, x1..x3: the solve outputs): ubar holds dL/du after the step on
+// entry and dL/du before it on return; fbar accumulates dL/dforcing.
+int bt_adj_step(npbmz_ctx* ctx, const double* us, const double* x1, const double* x2,
+                const double* x3, double* ubar, double* lam, double* fbar) {
+  auto& c = *ctx;
+  int rc;
+  if ((rc = k_adj_solve<2>(ctx, us, x3, ubar, lam, ubar))) return rc;
+  if ((rc = k_adj_solve<1>(ctx, us, x2, lam, lam, ubar))) return rc;
+  if ((rc = k_adj_solve<0>(ctx, us, x1, lam, lam, ubar))) return rc;
+  const Table& t = c.t_all;
+  hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us, lam,
+                     ubar, fbar, c.zd, t.d, c.K);
+  CK(hipGetLastError());
+  if (c.nz_ > 1 && c.t_face.n > 0) {
+    for (int xf = 1; xf >= 0; --xf) {
+      hipLaunchKernelGGL(fast::bt::adj_exch_kernel<kExchNT>, dim3(c.t_face.n), dim3(kExchNT), 0,
+                         c.stream, ubar, c.zd, c.faces, c.t_face.d, xf);
+      CK(hipGetLastError());
+    }
+  }
+  return NPBMZ_OK;
+}
+
+// Sum of w*u^2 over every grid point of the current u (wd on the device);
+// with seed, also ubar = 2 w u and fbar = 0.
+int bt_loss(npbmz_ctx* ctx, const double* wd, double* ubar, double* fbar, bool seed, double* loss) {
+  auto& c = *ctx;
+  const Table& t = c.t_all;
+  std::vector<double> part((size_t)t.n);
+  hipLaunchKernelGGL(fast::bt::loss_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, c.u, wd,
+                     ubar, fbar, c.zd, t.d, c.partial, seed ? 1 : 0);
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(part.data(), c.partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                    c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (double v : part) s += v;
+  *loss = s;
   return NPBMZ_OK;
 }
 
@@ -1173,6 +1251,100 @@ int npbmz_size(const npbmz_ctx* ctx, int32_t* nzones, int64_t* npoints) {
   if (nzones) *nzones = ctx->nz_;
   if (npoints) *npoints = ctx->npoints;
   return NPBMZ_OK;
+}
+
+int npbmz_loss(npbmz_ctx* ctx, const double* w, double* loss) {
+  if (!ctx) return NPBMZ_EINVAL;
+  if (!w || !loss) return fail(ctx, NPBMZ_EINVAL, "null argument");
+  auto& c = *ctx;
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  double* wd = nullptr;
+  CK(hipMalloc(&wd, fb));
+  int rc = NPBMZ_OK;
+  if (hipMemcpyAsync(wd, w, fb, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+    rc = fail(ctx, NPBMZ_EHIP, "loss weights upload");
+  if (rc == NPBMZ_OK) rc = bt_loss(ctx, wd, nullptr, nullptr, false, loss);
+  (void)hipFree(wd);
+  return rc;
+}
+
+int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const double* w,
+                  double* grad_u0, double* grad_forcing, double* loss, double* fwd_ms,
+                  double* adj_ms) {
+  if (!ctx) return NPBMZ_EINVAL;
+  auto& c = *ctx;
+  if (c.bench != NPBMZ_BT) return fail(ctx, NPBMZ_ENOTSUP, "the adjoint is built for BT");
+  if (nsteps < 1 || ckpt_every < 1) return fail(ctx, NPBMZ_EINVAL, "nsteps and ckpt_every must be >= 1");
+  if (!w || !grad_u0 || !grad_forcing || !loss) return fail(ctx, NPBMZ_EINVAL, "null argument");
+  const size_t nd = 5 * (size_t)c.npoints, fb = sizeof(double) * nd;
+  const int k = std::min(ckpt_every, nsteps);
+  const int ncp = (nsteps + k - 1) / k;
+  // device buffers: w, ubar, lam, fbar, the checkpoints, and per step of a
+  // segment the post-exchange u and the three solve outputs
+  std::vector<double*> bufs;
+  auto alloc = [&](double** p) -> bool {
+    if (hipMalloc(p, fb) != hipSuccess) return false;
+    bufs.push_back(*p);
+    return true;
+  };
+  auto release = [&]() {
+    for (double* p : bufs) (void)hipFree(p);
+  };
+  double *wd, *ubar, *lam, *fbar;
+  std::vector<double*> ck(ncp), us(k), x1(k), x2(k), x3(k);
+  bool ok = alloc(&wd) && alloc(&ubar) && alloc(&lam) && alloc(&fbar);
+  for (int q = 0; q < ncp && ok; ++q) ok = alloc(&ck[q]);
+  for (int q = 0; q < k && ok; ++q) ok = alloc(&us[q]) && alloc(&x1[q]) && alloc(&x2[q]) && alloc(&x3[q]);
+  if (!ok) {
+    release();
+    return fail(ctx, NPBMZ_ENOMEM, "adjoint buffers");
+  }
+  int rc = NPBMZ_OK;
+  auto hip = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == NPBMZ_OK) rc = fail(ctx, NPBMZ_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hip(hipMemcpyAsync(wd, w, fb, hipMemcpyHostToDevice, c.stream), "weights upload");
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (auto& x : e) hip(hipEventCreate(&x), "hipEventCreate");
+  // forward pass, keeping u every k steps
+  if (rc == NPBMZ_OK) hip(hipEventRecord(e[0], c.stream), "record");
+  for (int n = 0; n < nsteps && rc == NPBMZ_OK; ++n) {
+    if (n % k == 0) hip(hipMemcpyAsync(ck[n / k], c.u, fb, hipMemcpyDeviceToDevice, c.stream), "checkpoint");
+    if (rc == NPBMZ_OK) rc = bt_step_store(ctx, nullptr, nullptr, nullptr, nullptr);
+  }
+  if (rc == NPBMZ_OK) hip(hipEventRecord(e[1], c.stream), "record");
+  if (rc == NPBMZ_OK) rc = bt_loss(ctx, wd, ubar, fbar, true, loss);
+  // reverse pass: recompute each segment from its checkpoint, then reverse
+  // its steps from the last to the first
+  if (rc == NPBMZ_OK) hip(hipEventRecord(e[2], c.stream), "record");
+  for (int sgm = ncp - 1; sgm >= 0 && rc == NPBMZ_OK; --sgm) {
+    const int n0 = sgm * k, n1 = std::min(nsteps, n0 + k);
+    hip(hipMemcpyAsync(c.u, ck[sgm], fb, hipMemcpyDeviceToDevice, c.stream), "restore");
+    for (int n = n0; n < n1 && rc == NPBMZ_OK; ++n)
+      rc = bt_step_store(ctx, us[n - n0], x1[n - n0], x2[n - n0], x3[n - n0]);
+    for (int n = n1 - 1; n >= n0 && rc == NPBMZ_OK; --n)
+      rc = bt_adj_step(ctx, us[n - n0], x1[n - n0], x2[n - n0], x3[n - n0], ubar, lam, fbar);
+  }
+  if (rc == NPBMZ_OK) hip(hipEventRecord(e[3], c.stream), "record");
+  // results, and u back to the initial state
+  if (rc == NPBMZ_OK) {
+    hip(hipMemcpyAsync(grad_u0, ubar, fb, hipMemcpyDeviceToHost, c.stream), "gradient download");
+    hip(hipMemcpyAsync(grad_forcing, fbar, fb, hipMemcpyDeviceToHost, c.stream), "gradient download");
+    hip(hipMemcpyAsync(c.u, ck[0], fb, hipMemcpyDeviceToDevice, c.stream), "restore u0");
+    hip(hipStreamSynchronize(c.stream), "synchronize");
+  }
+  if (rc == NPBMZ_OK) {
+    float a = 0.f, b = 0.f;
+    hip(hipEventElapsedTime(&a, e[0], e[1]), "elapsed");
+    hip(hipEventElapsedTime(&b, e[2], e[3]), "elapsed");
+    if (fwd_ms) *fwd_ms = a;
+    if (adj_ms) *adj_ms = b;
+  }
+  (void)hipStreamSynchronize(c.stream);
+  for (auto& x : e)
+    if (x) (void)hipEventDestroy(x);
+  release();
+  return rc;
 }
 
 int npbmz_probe_copy(npbmz_ctx* ctx, int64_t nbytes, int32_t reps, double* ms_per_launch) {

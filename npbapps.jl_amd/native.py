@@ -31,7 +31,7 @@ EXPORTS = (
     "npbmz_create", "npbmz_destroy", "npbmz_last_error", "npbmz_put_field",
     "npbmz_get_field", "npbmz_init_exact", "npbmz_step", "npbmz_run_phase", "npbmz_phase_times",
     "npbmz_norms", "npbmz_zone_norms", "npbmz_tune", "npbmz_describe", "npbmz_size",
-    "npbmz_probe_copy", "npbmz_set_schedule",
+    "npbmz_probe_copy", "npbmz_set_schedule", "npbmz_adjoint", "npbmz_loss",
 )
 
 
@@ -85,6 +85,8 @@ def load(path=LIB_PATH):
     lib.npbmz_size.argtypes = [P, ctypes.POINTER(i32), i64p]
     lib.npbmz_probe_copy.argtypes = [P, i64, i32, dp]
     lib.npbmz_set_schedule.argtypes = [P, i32, i32]
+    lib.npbmz_adjoint.argtypes = [P, i32, i32, dp, dp, dp, dp, dp, dp]
+    lib.npbmz_loss.argtypes = [P, dp, dp]
     for f in EXPORTS:
         if f not in ("npbmz_destroy", "npbmz_last_error"):
             getattr(lib, f).restype = ctypes.c_int
@@ -189,6 +191,24 @@ class Context:
         self.put_field(zone, field, arr, (1, org, org, org),
                        (5, nx - 1 + org, ny - 1 + org, nz - 1 + org))
 
+    @property
+    def size(self):
+        """Values in a whole-run field: every zone's nx*ny*nz*5, zone after zone."""
+        return 5 * sum(int(a) * int(b) * int(c)
+                       for a, b, c in zip(self.zones.nx, self.zones.ny, self.zones.nz))
+
+    def get_all(self, field):
+        """Every zone's field, zone after zone, each (nz, ny, nx, 5) flattened."""
+        return np.concatenate([self.get_dense(z, field).ravel() for z in range(self.zones.count)])
+
+    def put_all(self, field, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float64)
+        off = 0
+        for z in range(self.zones.count):
+            n = 5 * self.zones.nx[z] * self.zones.ny[z] * self.zones.nz[z]
+            self.put_dense(z, field, flat[off:off + n])
+            off += n
+
     # -- compute -------------------------------------------------------------
     def init_exact(self):
         self._check(self._lib.npbmz_init_exact(self._h), "npbmz_init_exact")
@@ -234,6 +254,31 @@ class Context:
     def set_schedule(self, which, value):
         self._check(self._lib.npbmz_set_schedule(self._h, int(which), int(value)),
                     "npbmz_set_schedule")
+
+    def loss(self, w):
+        """sum(w * u**2) over every grid point of the current u (get_all layout)."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.size
+        out = ctypes.c_double(0.0)
+        self._check(self._lib.npbmz_loss(self._h, _dptr(w), ctypes.byref(out)), "npbmz_loss")
+        return out.value
+
+    def adjoint(self, nsteps, ckpt_every, w):
+        """BT only: loss = sum(w*u_N**2) after nsteps steps from the current
+        state, and its gradients w.r.t. the initial u and the forcing (both in
+        get_all layout), with u kept every ckpt_every steps.  Returns a dict
+        with loss, grad_u0, grad_forcing and the forward / reverse times in
+        seconds; u is left at its initial value."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.size
+        gu = np.zeros(self.size)
+        gf = np.zeros(self.size)
+        loss, tf, ta = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.npbmz_adjoint(self._h, int(nsteps), int(ckpt_every), _dptr(w),
+                                            _dptr(gu), _dptr(gf), ctypes.byref(loss),
+                                            ctypes.byref(tf), ctypes.byref(ta)), "npbmz_adjoint")
+        return {"loss": loss.value, "grad_u0": gu, "grad_forcing": gf,
+                "fwd_s": tf.value * 1e-3, "adj_s": ta.value * 1e-3}
 
     def probe_copy(self, nbytes, reps=20):
         """Seconds per launch of a plain nbytes device-to-device copy kernel."""

@@ -15,7 +15,7 @@
 #include <cstring>
 #include <vector>
 
-
+#include "dual.h"
 #include "zone_ops.h"
 
 namespace npb {
@@ -475,6 +475,74 @@ void oracle_bt_line_blocks(void* h, int z, int dir, int a, int b, double* A, dou
         B[l * 25 + m * 5 + q] = s.b(m, q, l);
         C[l * 25 + m * 5 + q] = (l >= 1 && l <= nl - 2) ? tmp.LC(m, q, idx[0], idx[1], idx[2]) : 0.0;
       }
+  }
+}
+
+// Reference gradient by forward-mode dual numbers (no counterpart in the
+// reference, which has no derivatives).  From the run's current state u0 and
+// forcing f, runs nsteps timesteps in Dual<4> arithmetic with u0 seeded
+// along vu[d] and f along vf[d] (d = 0..ndir-1, four at a time) and
+// evaluates the loss L = sum over every grid point of w*u^2 of the final u.
+// w, vu, vf hold every zone's nx*ny*nz*5 values, zone after zone, each in the
+// dense [k][j][i][m] order of oracle_bt_get; vf is read at interior points
+// only (forcing is used nowhere else).  out[0] = L, out[1 + d] = the
+// directional derivative of L along (vu[d], vf[d]).  The run's state is left
+// unchanged.
+void oracle_bt_dual(void* h, int nsteps, int ndir, const double* w, const double* vu,
+                    const double* vf, double* out) {
+  constexpr int ND = 4;
+  using D = Dual<ND>;
+  Run* r = (Run*)h;
+  const int nzn = r->G.count();
+  std::vector<size_t> zoff(nzn + 1, 0);
+  for (int z = 0; z < nzn; ++z)
+    zoff[z + 1] = zoff[z] + (size_t)5 * r->Z[z].nx * r->Z[z].ny * r->Z[z].nz;
+  for (int d0 = 0; d0 < std::max(ndir, 1); d0 += ND) {
+    std::vector<Zone<D>> Z(nzn);
+    for (int z = 0; z < nzn; ++z) {
+      Zone<double>& a = r->Z[z];
+      Zone<D>& b = Z[z];
+      b.nx = a.nx; b.ny = a.ny; b.nz = a.nz;
+      b.alloc();
+      for (size_t q = 0; q < a.u.size(); ++q) b.u[q] = D(a.u[q]);
+      for (size_t q = 0; q < a.forcing.size(); ++q) b.forcing[q] = D(a.forcing[q]);
+      size_t p = zoff[z];
+      for (int k = 0; k < a.nz; ++k)
+        for (int j = 0; j < a.ny; ++j)
+          for (int i = 0; i < a.nx; ++i)
+            for (int m = 0; m < 5; ++m, ++p)
+              for (int e = 0; e < ND && d0 + e < ndir; ++e) {
+                const size_t off = (size_t)(d0 + e) * zoff[nzn];
+                b.U(m, i, j, k).d[e] = vu[off + p];
+                const bool in = i > 0 && i < a.nx - 1 && j > 0 && j < a.ny - 1 && k > 0 && k < a.nz - 1;
+                if (in) b.F(m, i, j, k).d[e] = vf[off + p];
+              }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      if (nzn > 1) exch_qbc(Z, r->G);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(r->nthreads)
+      for (int z = 0; z < nzn; ++z) {
+        compute_rhs(Z[z], r->K);
+        solve_dir(Z[z], r->K, 0);
+        solve_dir(Z[z], r->K, 1);
+        solve_dir(Z[z], r->K, 2);
+        add(Z[z]);
+      }
+    }
+    D L(0.0);
+    for (int z = 0; z < nzn; ++z) {
+      Zone<D>& b = Z[z];
+      size_t p = zoff[z];
+      for (int k = 0; k < b.nz; ++k)
+        for (int j = 0; j < b.ny; ++j)
+          for (int i = 0; i < b.nx; ++i)
+            for (int m = 0; m < 5; ++m, ++p) {
+              const D& x = b.U(m, i, j, k);
+              L = L + w[p] * (x * x);
+            }
+    }
+    out[0] = L.v;
+    for (int e = 0; e < ND && d0 + e < ndir; ++e) out[1 + d0 + e] = L.d[e];
   }
 }
 

@@ -302,7 +302,8 @@ void compute_rhs(Zone<T>& z, const Consts& K, bool with_speed = false) {
         z.qs[d] = z.square[d] * rho_inv;
         if (with_speed) {  // src/SP/rhs.jl:85-88
           T aux = K.c1c2 * rho_inv * (z.U(4, i, j, k) - z.square[d]);
-          aux = std::sqrt(aux);
+          using std::sqrt;  // a dual scalar brings its own sqrt
+          aux = sqrt(aux);
           z.speed[d] = aux;
           z.ainv[d] = 1.0 / aux;
         }
