@@ -185,3 +185,41 @@ def test_repo_flop_counts():
     assert abs(P.mflops_per_step("SP", z) - 184.425) < 5e-3
     z, _ = P.zones_for("LU", "A")
     assert abs(P.mflops_per_step("LU", z) - 410.53) < 5e-2
+
+
+def test_dual_numbers_match_central_differences():
+    """The oracle's forward-mode dual run (the adjoint's reference gradient)
+    against central differences of its own double-precision run: class S,
+    5 steps, two directions, the loss difference summed point by point.
+    h = 1e-4: rounding noise (which grows as 1/h) and truncation (as h^2)
+    are both below 1e-8 there."""
+    zones, p = P.zones_for("BT", "S")
+    o = OracleBT.for_class(p)
+    r = np.random.Generator(np.random.Philox(11))
+    shapes = [(zones.nz[z], zones.ny[z], zones.nx[z], 5) for z in range(zones.count)]
+    n = sum(int(np.prod(s)) for s in shapes)
+    w = r.random(n)
+    mask = np.concatenate([np.pad(np.ones((a - 2, b - 2, c - 2, 5)),
+                                  ((1, 1), (1, 1), (1, 1), (0, 0))).ravel() for a, b, c, _ in shapes])
+    vu = r.standard_normal((2, n))
+    vf = r.standard_normal((2, n)) * mask
+    L, d = o.dual_gradient(5, w, vu, vf)
+    u0 = [o.get(z, 0) for z in range(zones.count)]
+    f0 = [o.get(z, 1) for z in range(zones.count)]
+
+    def final(du, df):
+        off = 0
+        for z, s in enumerate(shapes):
+            k = int(np.prod(s))
+            o.set(z, 0, u0[z] + du[off:off + k].reshape(s))
+            o.set(z, 1, f0[z] + df[off:off + k].reshape(s))
+            off += k
+        o.step(5)
+        return np.concatenate([o.get(z, 0).ravel() for z in range(zones.count)])
+
+    h = 1e-4
+    for q in range(2):
+        up, um = final(h * vu[q], h * vf[q]), final(-h * vu[q], -h * vf[q])
+        fd = np.sum(w * (up - um) * (up + um)) / (2 * h)
+        assert abs(fd / d[q] - 1) < 1e-7, (fd, d[q])
+    o.close()
