@@ -58,6 +58,42 @@ def cpu_baseline(params, budget_s, threads):
                       f"{threads} OpenMP threads over zones, {t:.2f} s"}
 
 
+def adjoint_case(P):
+    """The adjoint case: BT-MZ class A (128x128x16, 4x4 uneven zones), 200
+    steps at dt=0.0008, loss sum(w u^2) with w uniform [0,1) (Philox seed 20),
+    checkpoints every 20 steps.  The reverse time includes recomputing every
+    segment from its checkpoint."""
+    import numpy as np
+    zones, p = P.zones_for("BT", "A")
+    ctx = P.Context("BT", zones, p.dt)
+    ctx.init_exact()
+    w = np.random.Generator(np.random.Philox(20)).random(ctx.size)
+    ctx.adjoint(p.niter, 20, w)  # warm-up
+    r = ctx.adjoint(p.niter, 20, w)
+    ctx.close()
+    return {"workload": "BT-MZ class A, 4x4 uneven zones, 200 steps, checkpoint every 20",
+            "forward_ms_per_step": round(r["fwd_s"] / p.niter * 1e3, 5),
+            "adjoint_ms_per_step": round(r["adj_s"] / p.niter * 1e3, 5),
+            "ratio": round(r["adj_s"] / r["fwd_s"], 3)}
+
+
+def class_run(P, bench, cls, npb_exchange=False):
+    """A whole class run (graph mode) and the same steps launched per zone."""
+    zones, p = P.zones_for(bench, cls)
+    r = P.go(bench, cls, mode="graph", verbose=False, npb_exchange=npb_exchange)
+    ctx = P.Context(bench, zones, p.dt, omega=1.2, npb_exchange=npb_exchange)
+    ctx.init_exact()
+    n = min(p.niter, 50)
+    ctx.step(2, "perzone")
+    t_pz = ctx.step(n, "perzone") / n
+    ctx.close()
+    t = r["time_s"] / p.niter
+    return {"mops": round(r["mops"], 2), "ms_per_step": round(t * 1e3, 5),
+            "grid_pt_steps_per_s": round(zones.npoints / t, 1), "verified": r["verified"],
+            "perzone_ms_per_step": round(t_pz * 1e3, 5), "graph_vs_perzone": round(t_pz / t, 3),
+            "schedule": r["schedule"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +102,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perzone", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the adjoint ratio and the SP/LU class A runs")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain copy kernel run for the achievable-bandwidth "
                          "reference and counter calibration (0 = skip)")
@@ -122,6 +160,12 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get(dom)
 
+    extra = {}
+    if not args.no_extra:
+        extra["adjoint"] = adjoint_case(P)
+        extra["sp_class_A"] = class_run(P, "SP", "A")
+        extra["lu_class_A"] = class_run(P, "LU", "A", npb_exchange=True)
+
     value = mflop_step * K / t_graph
     out = {
         "metric": "Mop/s & grid-pt-steps/s FP64 1 GPU: batched vs per-zone; adjoint/forward ratio",
@@ -147,7 +191,10 @@ def main():
             "value": round(mflop_step * K / t_perzone, 2), "unit": "Mop/s",
             "ms_per_step": round(t_perzone / K * 1e3, 5)},
         "graph_vs_perzone": None if t_perzone is None else round(t_perzone / t_graph, 3),
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": extra.get("adjoint", {}).get("ratio"),
+        "adjoint": extra.get("adjoint"),
+        "sp_class_A": extra.get("sp_class_A"),
+        "lu_class_A": extra.get("lu_class_A"),
         "phase_ms_per_step": {n: round(per_launch_s[n] * 1e3, 5) for n in PHASES},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
