@@ -163,6 +163,8 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 #define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone */
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
+#define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
+                                    with a builder and an eliminator wave */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -140,6 +140,7 @@ struct npbmz_ctx {
   size_t cscr_slot = 0;  // doubles per scratch slot (max over directions)
   int maxrows[3] = {0, 0, 0};
   int solve_grid[3] = {0, 0, 0};  // 0: one block per task, else persistent grid
+  int solve_pair = 0;             // BT: bit d set: direction d uses solve_pair_kernel
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
   Table t_int, t_all, t_line[3], t_face, t_face2;
@@ -305,8 +306,9 @@ int build_tables(npbmz_ctx* ctx) {
     for (int z = 0; z < c.nz_; ++z) slots = std::max(slots, c.t_line[d].zcount[z]);
     if (c.solve_grid[d] > 0) slots = std::max(slots, c.solve_grid[d]);
     // reduced coefficients kept per row for the back sweep: BT's 5x5 C'
-    // block, SP's lhs(4:5) of three systems
-    const int per_row = c.bench == NPBMZ_SP ? 6 : 25;
+    // block and reduced rhs (fast::bt::kRowPairs double pairs), SP's
+    // lhs(4:5) of three systems
+    const int per_row = c.bench == NPBMZ_SP ? 6 : 2 * fast::bt::kRowPairs;
     slot = std::max(slot, (size_t)slots * mr * per_row * 64);
   }
   if (slot > c.cscr_slot) {
@@ -416,16 +418,19 @@ int k_solve_t(npbmz_ctx* ctx, int zone) {
   if (n <= 0) return NPBMZ_OK;
   int grid = n;
   if (zone < 0 && c.solve_grid[DIR] > 0) grid = std::min(n, c.solve_grid[DIR]);
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kLineNT), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
+  auto go = [&](auto kern, int nt) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
                        t.d + first, n, c.maxrows[DIR], c.K);
   };
   if (c.bench == NPBMZ_SP) {
-    if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>);
-    else go(fast::sp::solve_kernel<DIR, FUSE>);
+    if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>, kLineNT);
+    else go(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
+  } else if ((c.solve_pair >> DIR) & 1) {
+    if (strict(ctx)) go(exact::bt::solve_pair_kernel<DIR, FUSE>, 2 * kLineNT);
+    else go(fast::bt::solve_pair_kernel<DIR, FUSE>, 2 * kLineNT);
   } else {
-    if (strict(ctx)) go(exact::bt::solve_kernel<DIR, FUSE>);
-    else go(fast::bt::solve_kernel<DIR, FUSE>);
+    if (strict(ctx)) go(exact::bt::solve_kernel<DIR, FUSE>, kLineNT);
+    else go(fast::bt::solve_kernel<DIR, FUSE>, kLineNT);
   }
   CK(hipGetLastError());
   return NPBMZ_OK;
@@ -706,9 +711,9 @@ void describe(npbmz_ctx* c) {
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else
-    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d fp=%s", c->nz_,
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d fp=%s", c->nz_,
              c->order.empty() || c->order[0] == 0 ? "natural" : "largest-first", c->solve_grid[0],
-             c->solve_grid[1], c->solve_grid[2], strict(c) ? "strict" : "fma");
+             c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? "strict" : "fma");
   c->sched = b;
 }
 
@@ -1224,6 +1229,11 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench != NPBMZ_LU || (value != 128 && value != 256 && value != 512))
         return fail(ctx, NPBMZ_EINVAL, "LU workgroup size is 128, 256 or 512");
       c.lu_nt = value;
+      break;
+    case NPBMZ_SCHED_SOLVE_PAIR:
+      if (c.bench != NPBMZ_BT || value < 0 || value > 7)
+        return fail(ctx, NPBMZ_EINVAL, "BT solve pair mask is 0..7");
+      c.solve_pair = value;
       break;
     case NPBMZ_SCHED_SOLVE_GRID: {
       if (c.bench == NPBMZ_LU || value < 0) return fail(ctx, NPBMZ_EINVAL, "bad solve grid");

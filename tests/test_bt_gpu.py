@@ -13,9 +13,14 @@ from oracle_lib import OracleBT
 pytestmark = pytest.mark.gpu
 
 
-def _ctx(P, cls, strict):
+def _ctx(P, cls, strict, pair=None):
+    """pair: bit d set solves direction d with the builder/eliminator wave
+    pair instead of one wave per 64 lines (None keeps the default)."""
     zones, p = P.zones_for("BT", cls)
-    return P.Context("BT", zones, p.dt, strict=strict), zones, p
+    ctx = P.Context("BT", zones, p.dt, strict=strict)
+    if pair is not None:
+        ctx.set_schedule(P.native.SCHED_SOLVE_PAIR, pair)
+    return ctx, zones, p
 
 
 def _load_oracle_state(ctx, o, zones):
@@ -40,12 +45,14 @@ def test_init_exact_is_bit_exact(gpu, cls):
     ctx.close()
 
 
+@pytest.mark.parametrize("pair", [0, 7])
 @pytest.mark.parametrize("cls", ["S", "W"])
 @pytest.mark.parametrize("mode", ["perzone", "batched"])
-def test_each_phase_is_bit_exact(gpu, cls, mode):
+def test_each_phase_is_bit_exact(gpu, cls, mode, pair):
     """exch_qbc, compute_rhs, x/y/z_solve and add, one at a time, from the
-    same state as the oracle (a few steps into the run)."""
-    ctx, zones, p = _ctx(gpu, cls, True)
+    same state as the oracle (a few steps into the run), with either line
+    solve kernel."""
+    ctx, zones, p = _ctx(gpu, cls, True, pair)
     o = OracleBT.for_class(p)
     o.step(3)
     _load_oracle_state(ctx, o, zones)
@@ -65,10 +72,11 @@ def test_each_phase_is_bit_exact(gpu, cls, mode):
     ctx.close()
 
 
-def test_full_run_strict_is_bit_exact_class_S(gpu):
+@pytest.mark.parametrize("pair", [0, 7])
+def test_full_run_strict_is_bit_exact_class_S(gpu, pair):
     """60 timesteps of class S: u identical to the oracle's in every zone;
     norms agree to the last bits of a differently ordered sum."""
-    ctx, zones, p = _ctx(gpu, "S", True)
+    ctx, zones, p = _ctx(gpu, "S", True, pair)
     o = OracleBT.for_class(p)
     ctx.init_exact()
     ctx.step(p.niter, "graph")
@@ -95,6 +103,23 @@ def test_modes_agree_bitwise(gpu):
     for r in res[1:]:
         for a, b in zip(res[0], r):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("cls", ["W", "A"])
+def test_line_solve_kernels_agree(gpu, cls):
+    """Strict arithmetic: the wave-pair and the one-wave line solves give the
+    same bits over 5 steps; FMA arithmetic: each kernel contracts its own
+    products, so they agree to 1e-12 relative."""
+    for strict, tol in ((True, 0.0), (False, 1e-12)):
+        out = []
+        for pair in (0, 7):
+            ctx, zones, p = _ctx(gpu, cls, strict, pair)
+            ctx.init_exact()
+            ctx.step(5, "graph")
+            out.append(ctx.get_all(0))
+            ctx.close()
+        err = np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
+        assert err <= tol, (strict, err)
 
 
 @pytest.mark.parametrize("cls", ["S", "W"])
