@@ -80,7 +80,7 @@ def adjoint_case(P):
 def class_run(P, bench, cls, npb_exchange=False):
     """A whole class run (graph mode) and the same steps launched per zone."""
     zones, p = P.zones_for(bench, cls)
-    r = P.go(bench, cls, mode="graph", verbose=False, npb_exchange=npb_exchange)
+    r = P.go(bench, cls, mode="graph", verbose=False, npb_exchange=npb_exchange, tune_ms=1000)
     ctx = P.Context(bench, zones, p.dt, omega=1.2, npb_exchange=npb_exchange)
     ctx.init_exact()
     n = min(p.niter, 50)
@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perzone", action="store_true")
+    ap.add_argument("--tune-ms", type=int, default=1500,
+                    help="autotune budget before timing (0 = default schedule)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint ratio and the SP/LU class A runs")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
@@ -121,6 +123,9 @@ def main():
     npts = zones.npoints
 
     ctx = P.Context("BT", zones, p.dt)
+    ctx.init_exact()
+    if args.tune_ms > 0:
+        ctx.tune(args.tune_ms)
     ctx.init_exact()
     ctx.step(W, "graph")
     ctx.init_exact()

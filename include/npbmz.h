@@ -146,15 +146,18 @@ int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci);
  * (nzones x 5 each, zone-major); either pointer may be NULL. */
 int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err);
 
-/* Autotune the batched launch schedule by timing candidate schedules for
- * about budget_ms in total: for BT/SP the zone order of the task tables
- * (largest first or natural) and the solve grid (one block per 64 lines, or
- * a persistent grid of 1024/2048 blocks striding over them); for LU the SSOR
- * sweep schedule (one launch per i+j+k hyperplane over all zones, or one
- * workgroup of 128/256/512 threads per zone walking all hyperplanes).  The
- * state is restored afterwards; the fastest schedule is kept in the context
- * for the batched and graph modes.  No counterpart in the reference (its
- * zone->thread schedule is Threads.@threads over zones,
+/* Autotune the launch schedule by timing graph replays of candidate
+ * schedules for about budget_ms in total, one knob after another (each knob
+ * keeps its fastest value while the next is tried): for BT/SP the task order
+ * (largest zones first, and in each solve direction the zones with the
+ * longest lines first; or natural order), the line-solve kernel per
+ * direction (BT: one wave per 64 lines, or a builder/eliminator wave pair)
+ * and the solve grid (one block per 64 lines, or a persistent grid of 1024
+ * blocks striding over them); for LU the SSOR sweep schedule (one workgroup
+ * of 128/256/512 threads per zone walking all hyperplanes, or one launch per
+ * i+j+k hyperplane over all zones).  The state is restored afterwards; the
+ * chosen schedule is kept in the context for all modes.  No counterpart in
+ * the reference (its zone->thread schedule is Threads.@threads over zones,
  * src/BT/bt-node.jl:318). */
 int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 

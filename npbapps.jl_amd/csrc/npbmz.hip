@@ -141,6 +141,7 @@ struct npbmz_ctx {
   int maxrows[3] = {0, 0, 0};
   int solve_grid[3] = {0, 0, 0};  // 0: one block per task, else persistent grid
   int solve_pair = 0;             // BT: bit d set: direction d uses solve_pair_kernel
+  bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
   Table t_int, t_all, t_line[3], t_face, t_face2;
@@ -293,7 +294,17 @@ int build_tables(npbmz_ctx* ctx) {
     return rc;
   size_t slot = 0;
   for (int d = 0; d < 3 && c.bench != NPBMZ_LU; ++d) {
-    if ((rc = build_table(ctx, c.t_line[d], c.order, kLineNT,
+    // Line tasks: with the largest-first order, zones with the longest lines
+    // first (then the most lines), so that when a direction has more tasks
+    // than the GPU holds at once the slowest ones start first.
+    std::vector<int> ord = c.order;
+    if (c.largest_first) {
+      auto len = [&](int z) { return d == 0 ? c.nx[z] : d == 1 ? c.ny[z] : c.nz[z]; };
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return len(a) != len(b) ? len(a) > len(b) : lines_of(ctx, a, d) > lines_of(ctx, b, d);
+      });
+    }
+    if ((rc = build_table(ctx, c.t_line[d], ord, kLineNT,
                           [&](int z) { return lines_of(ctx, z, d); })))
       return rc;
     int mr = 0;
@@ -712,7 +723,7 @@ void describe(npbmz_ctx* c) {
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else
     snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d fp=%s", c->nz_,
-             c->order.empty() || c->order[0] == 0 ? "natural" : "largest-first", c->solve_grid[0],
+             c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
              c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? "strict" : "fma");
   c->sched = b;
 }
@@ -1148,36 +1159,45 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   if (!ctx) return NPBMZ_EINVAL;
   if (budget_ms < 0) return fail(ctx, NPBMZ_EINVAL, "budget_ms < 0");
   auto& c = *ctx;
-  // Candidate schedules.  LU: hyperplane launches, or one workgroup per zone
-  // of 128/256/512 threads.  BT/SP: zone order of the task tables x solve
-  // grid (one block per task, or a persistent grid that strides over them).
-  struct Cand { int a, b; };
-  std::vector<Cand> cands;
+  // Schedule knobs, tuned one after another: each keeps its fastest value
+  // while the next is tried.  LU: the sweep form (hyperplane launches, or one
+  // workgroup of 128/256/512 threads per zone).  BT/SP: the task order
+  // (largest zones and longest lines first, or natural), the line-solve
+  // kernel per direction (BT) and the solve grid.  Trial steps run as graph
+  // replays; the state is saved and restored around them.
+  struct Knob { int which; std::vector<int> values; };
+  std::vector<Knob> knobs;
   if (c.bench == NPBMZ_LU) {
-    cands = {{kSweepPlane, 0}, {kSweepZone, 128}, {kSweepZone, 256}, {kSweepZone, 512}};
+    knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}}};
   } else {
-    for (int ord = 0; ord < 2; ++ord)
-      for (int g : {0, 1024, 2048}) cands.push_back({ord, g});
+    knobs.push_back({-1, {1, 0}});
+    if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
+    knobs.push_back({NPBMZ_SCHED_SOLVE_GRID, {0, 1024}});
   }
-  auto apply = [&](const Cand& k) -> int {
-    if (c.bench == NPBMZ_LU) {
-      c.lu_sweep = k.a;
-      c.lu_nt = k.b ? k.b : c.lu_nt;
-    } else {
-      std::iota(c.order.begin(), c.order.end(), 0);
-      if (k.a == 0)
-        std::stable_sort(c.order.begin(), c.order.end(), [&](int x, int y) {
-          return (int64_t)c.nx[x] * c.ny[x] * c.nz[x] > (int64_t)c.nx[y] * c.ny[y] * c.nz[y];
-        });
-      for (int d = 0; d < 3; ++d) c.solve_grid[d] = k.b;
-      int rc = build_tables(ctx);
-      if (rc) return rc;
+  auto get = [&](int which) -> int {
+    switch (which) {
+      case NPBMZ_SCHED_LU_SWEEP: return c.lu_sweep;
+      case NPBMZ_SCHED_LU_THREADS: return c.lu_nt;
+      case NPBMZ_SCHED_SOLVE_PAIR: return c.solve_pair;
+      case NPBMZ_SCHED_SOLVE_GRID: return c.solve_grid[0];
+      default: return c.largest_first ? 1 : 0;
     }
+  };
+  auto set = [&](int which, int v) -> int {
+    if (which >= 0) {
+      if (which == NPBMZ_SCHED_LU_THREADS && c.lu_sweep != kSweepZone) return NPBMZ_OK;
+      return npbmz_set_schedule(ctx, which, v);
+    }
+    c.largest_first = v == 1;
+    std::iota(c.order.begin(), c.order.end(), 0);
+    if (c.largest_first)
+      std::stable_sort(c.order.begin(), c.order.end(), [&](int x, int y) {
+        return (int64_t)c.nx[x] * c.ny[x] * c.nz[x] > (int64_t)c.nx[y] * c.ny[y] * c.nz[y];
+      });
     if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
     if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
-    return NPBMZ_OK;
+    return build_tables(ctx);
   };
-  // the state is saved and restored around the trial steps
   const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
   double *su = nullptr, *sr = nullptr;
   CK(hipMalloc(&su, fb));
@@ -1186,28 +1206,35 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   if (hipMemcpyAsync(su, c.u, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
       hipMemcpyAsync(sr, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
     rc = fail(ctx, NPBMZ_EHIP, "tune save");
-  const double per = std::max(1.0, double(budget_ms) / cands.size());
-  double best = 1e300;
-  size_t bi = 0;
-  for (size_t q = 0; q < cands.size() && rc == NPBMZ_OK; ++q) {
-    if ((rc = apply(cands[q]))) break;
+  size_t trials = 0;
+  for (auto& k : knobs) trials += k.values.size();
+  const double per = std::max(1.0, double(budget_ms) / trials);
+  auto trial = [&](double& t) -> int {
     double ms = 0.0, tot = 0.0;
-    int steps = 0;
-    if ((rc = npbmz_step(ctx, 1, NPBMZ_MODE_BATCHED, &ms))) break;  // warm-up
-    tot = 0.0;
+    int steps = 0, r;
+    if ((r = npbmz_step(ctx, 1, NPBMZ_MODE_GRAPH, &ms))) return r;  // capture + warm-up
     do {
-      if ((rc = npbmz_step(ctx, 1, NPBMZ_MODE_BATCHED, &ms))) break;
+      if ((r = npbmz_step(ctx, 1, NPBMZ_MODE_GRAPH, &ms))) return r;
       tot += ms;
       ++steps;
     } while (tot < per && steps < 50);
-    if (rc) break;
-    const double t = tot / steps;
-    if (t < best) { best = t; bi = q; }
+    t = tot / steps;
     if (hipMemcpyAsync(c.u, su, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
         hipMemcpyAsync(c.rhs, sr, fb, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
-      rc = fail(ctx, NPBMZ_EHIP, "tune restore");
+      return fail(ctx, NPBMZ_EHIP, "tune restore");
+    return NPBMZ_OK;
+  };
+  for (auto& k : knobs) {
+    if (rc) break;
+    int best_v = get(k.which);
+    double best = 1e300;
+    for (int v : k.values) {
+      double t = 0.0;
+      if ((rc = set(k.which, v)) || (rc = trial(t))) break;
+      if (t < best) { best = t; best_v = v; }
+    }
+    if (rc == NPBMZ_OK) rc = set(k.which, best_v);
   }
-  if (rc == NPBMZ_OK) rc = apply(cands[bi]);
   if (hipStreamSynchronize(c.stream) != hipSuccess && rc == NPBMZ_OK)
     rc = fail(ctx, NPBMZ_EHIP, "tune sync");
   (void)hipFree(su);
