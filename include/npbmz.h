@@ -151,7 +151,8 @@ int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err);
  * keeps its fastest value while the next is tried): for BT/SP the task order
  * (largest zones first, and in each solve direction the zones with the
  * longest lines first; or natural order), the line-solve kernel per
- * direction (BT: one wave per 64 lines, or a builder/eliminator wave pair)
+ * direction (BT: one wave per 64 lines, a builder/eliminator wave pair, or,
+ * with FMA arithmetic, the two-sided elimination)
  * and the solve grid (one block per 64 lines, or a persistent grid of 1024
  * blocks striding over them); for LU the SSOR sweep schedule (one workgroup
  * of 128/256/512 threads per zone walking all hyperplanes, or one launch per
@@ -168,6 +169,10 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
 #define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
                                     with a builder and an eliminator wave */
+#define NPBMZ_SCHED_SOLVE_TWIST 4 /* BT, FMA arithmetic only (ignored with
+                                     NPBMZ_FLAG_STRICT_FP): bit d set: direction d
+                                     eliminates each line from both ends with two
+                                     waves meeting in the middle (overrides PAIR) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -155,6 +155,7 @@ struct npbmz_ctx {
   int maxrows[3] = {0, 0, 0};
   int solve_grid[3] = {0, 0, 0};  // 0: one block per task, else persistent grid
   int solve_pair = 0;             // BT: bit d set: direction d uses solve_pair_kernel
+  int solve_twist = 0;            // BT, FMA only: bit d set: solve_twist_kernel (wins over pair)
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
@@ -450,6 +451,8 @@ int k_solve_t(npbmz_ctx* ctx, int zone) {
   if (c.bench == NPBMZ_SP) {
     if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>, kLineNT);
     else go(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
+  } else if (!strict(ctx) && ((c.solve_twist >> DIR) & 1)) {
+    go(fast::bt::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
   } else if ((c.solve_pair >> DIR) & 1) {
     if (strict(ctx)) go(exact::bt::solve_pair_kernel<DIR, FUSE>, 2 * kLineNT);
     else go(fast::bt::solve_pair_kernel<DIR, FUSE>, 2 * kLineNT);
@@ -736,9 +739,10 @@ void describe(npbmz_ctx* c) {
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else
-    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d fp=%s", c->nz_,
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d fp=%s", c->nz_,
              c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
-             c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? "strict" : "fma");
+             c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? 0 : c->solve_twist,
+             strict(c) ? "strict" : "fma");
   c->sched = b;
 }
 
@@ -1186,6 +1190,8 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
+    if (c.bench == NPBMZ_BT && !strict(ctx))
+      knobs.push_back({NPBMZ_SCHED_SOLVE_TWIST, {0, 1, 3, 7}});
     knobs.push_back({NPBMZ_SCHED_SOLVE_GRID, {0, 1024}});
   }
   auto get = [&](int which) -> int {
@@ -1193,6 +1199,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
       case NPBMZ_SCHED_LU_SWEEP: return c.lu_sweep;
       case NPBMZ_SCHED_LU_THREADS: return c.lu_nt;
       case NPBMZ_SCHED_SOLVE_PAIR: return c.solve_pair;
+      case NPBMZ_SCHED_SOLVE_TWIST: return c.solve_twist;
       case NPBMZ_SCHED_SOLVE_GRID: return c.solve_grid[0];
       default: return c.largest_first ? 1 : 0;
     }
@@ -1275,6 +1282,11 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench != NPBMZ_BT || value < 0 || value > 7)
         return fail(ctx, NPBMZ_EINVAL, "BT solve pair mask is 0..7");
       c.solve_pair = value;
+      break;
+    case NPBMZ_SCHED_SOLVE_TWIST:
+      if (c.bench != NPBMZ_BT || value < 0 || value > 7)
+        return fail(ctx, NPBMZ_EINVAL, "BT two-sided solve mask is 0..7");
+      c.solve_twist = value;
       break;
     case NPBMZ_SCHED_SOLVE_GRID: {
       if (c.bench == NPBMZ_LU || value < 0) return fail(ctx, NPBMZ_EINVAL, "bad solve grid");

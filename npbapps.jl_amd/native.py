@@ -24,7 +24,7 @@ MODES = {"perzone": MODE_PERZONE, "batched": MODE_BATCHED, "graph": MODE_GRAPH}
 PHASE_EXCH, PHASE_RHS, PHASE_XSOLVE, PHASE_YSOLVE, PHASE_ZSOLVE, PHASE_ADD = range(6)
 # LU's phases (exch and rhs as above)
 PHASE_SSOR_LOWER, PHASE_SSOR_UPPER, PHASE_SSOR_UPDATE = 2, 3, 4
-SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID, SCHED_SOLVE_PAIR = 0, 1, 2, 3
+SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID, SCHED_SOLVE_PAIR, SCHED_SOLVE_TWIST = 0, 1, 2, 3, 4
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (

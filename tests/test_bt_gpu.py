@@ -72,11 +72,13 @@ def test_each_phase_is_bit_exact(gpu, cls, mode, pair):
     ctx.close()
 
 
-@pytest.mark.parametrize("pair", [0, 7])
-def test_full_run_strict_is_bit_exact_class_S(gpu, pair):
+@pytest.mark.parametrize("pair,twist", [(0, 0), (7, 0), (0, 7)])
+def test_full_run_strict_is_bit_exact_class_S(gpu, pair, twist):
     """60 timesteps of class S: u identical to the oracle's in every zone;
-    norms agree to the last bits of a differently ordered sum."""
+    norms agree to the last bits of a differently ordered sum.  Strict
+    arithmetic ignores the two-sided solve (it reorders the elimination)."""
     ctx, zones, p = _ctx(gpu, "S", True, pair)
+    ctx.set_schedule(gpu.native.SCHED_SOLVE_TWIST, twist)
     o = OracleBT.for_class(p)
     ctx.init_exact()
     ctx.step(p.niter, "graph")
@@ -136,6 +138,43 @@ def test_fma_kernels_track_oracle_per_step(gpu, cls):
         oxcr, oxce = o.norms()
         assert np.abs(gxcr / oxcr - 1).max() < 1e-12, (s, gxcr, oxcr)
         assert np.abs(gxce / oxce - 1).max() < 1e-12, (s, gxce, oxce)
+    ctx.close()
+
+
+@pytest.mark.parametrize("cls", ["S", "W", "A"])
+def test_two_sided_solve_tracks_oracle(gpu, cls):
+    """The two-sided elimination in all three directions (FMA arithmetic):
+    rsdnrm/errnorm within 1e-12 of the oracle after each of the first 10
+    steps; the whole class run passes the reference's verification and ends
+    within 1e-10 of the oracle; and 5 steps agree with the one-wave solve to
+    1e-12."""
+    N = gpu.native
+    ctx, zones, p = _ctx(gpu, cls, False)
+    ctx.set_schedule(N.SCHED_SOLVE_TWIST, 7)
+    o = OracleBT.for_class(p, threads=16)
+    ctx.init_exact()
+    for s in range(10):
+        ctx.step(1, "graph")
+        o.step(1)
+        gxcr, gxce, _ = ctx.norms()
+        oxcr, oxce = o.norms()
+        assert np.abs(gxcr / oxcr - 1).max() < 1e-12, (s, gxcr, oxcr)
+        assert np.abs(gxce / oxce - 1).max() < 1e-12, (s, gxce, oxce)
+    ctx.step(p.niter - 10, "graph")
+    o.step(p.niter - 10)
+    gxcr, gxce, _ = ctx.norms()
+    oxcr, oxce = o.norms()
+    ok, rel = gpu.verify.verify("BT", cls, p.dt, p.niter, gxcr, gxce)
+    assert ok, rel
+    assert np.abs(gxcr / oxcr - 1).max() < 1e-10
+    assert np.abs(gxce / oxce - 1).max() < 1e-10
+    out = []
+    for twist in (0, 7):
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, twist)
+        ctx.init_exact()
+        ctx.step(5, "graph")
+        out.append(ctx.get_all(0))
+    assert np.abs(out[0] - out[1]).max() / np.abs(out[0]).max() < 1e-12
     ctx.close()
 
 
