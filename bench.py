@@ -102,8 +102,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perzone", action="store_true")
-    ap.add_argument("--tune-ms", type=int, default=1500,
-                    help="autotune budget before timing (0 = default schedule)")
+    ap.add_argument("--tune-ms", type=int, default=0,
+                    help="autotune budget before timing (0: the library's default schedule, "
+                         "the fastest the tuner finds for this class on MI355X)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint ratio and the SP/LU class A runs")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,

@@ -168,11 +168,12 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
 #define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
-                                    with a builder and an eliminator wave */
+                                    with a builder and an eliminator wave (default 2) */
 #define NPBMZ_SCHED_SOLVE_TWIST 4 /* BT, FMA arithmetic only (ignored with
                                      NPBMZ_FLAG_STRICT_FP): bit d set: direction d
                                      eliminates each line from both ends with two
-                                     waves meeting in the middle (overrides PAIR) */
+                                     waves meeting in the middle (overrides PAIR;
+                                     default 7) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -154,8 +154,12 @@ struct npbmz_ctx {
   size_t cscr_slot = 0;  // doubles per scratch slot (max over directions)
   int maxrows[3] = {0, 0, 0};
   int solve_grid[3] = {0, 0, 0};  // 0: one block per task, else persistent grid
-  int solve_pair = 0;             // BT: bit d set: direction d uses solve_pair_kernel
-  int solve_twist = 0;            // BT, FMA only: bit d set: solve_twist_kernel (wins over pair)
+  // BT line-solve kernels per direction.  The defaults are the fastest
+  // measured on MI355X for BT-MZ classes A and B: the two-sided solve in all
+  // three directions with FMA arithmetic; with strict arithmetic (no
+  // two-sided solve) the wave pair in y.
+  int solve_pair = 2;             // BT: bit d set: direction d uses solve_pair_kernel
+  int solve_twist = 7;            // BT, FMA only: bit d set: solve_twist_kernel (wins over pair)
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
@@ -738,6 +742,10 @@ void describe(npbmz_ctx* c) {
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
+  else if (c->bench == NPBMZ_SP)
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d fp=%s", c->nz_,
+             c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
+             c->solve_grid[1], c->solve_grid[2], strict(c) ? "strict" : "fma");
   else
     snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d fp=%s", c->nz_,
              c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
