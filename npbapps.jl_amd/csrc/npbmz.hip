@@ -110,6 +110,7 @@ __device__ __forceinline__ double recip(double x) {
 #include "bt_kernels.inc"
 #include "bt_adjoint.inc"
 #include "sp_kernels.inc"
+#include "sp_adjoint.inc"
 #include "lu_kernels.inc"
 }  // namespace fast
 
@@ -656,6 +657,94 @@ int bt_adj_step(npbmz_ctx* ctx, const double* us, const double* x1, const double
   if ((rc = k_adj_solve<2>(ctx, us, x3, ubar, lam, ubar))) return rc;
   if ((rc = k_adj_solve<1>(ctx, us, x2, lam, lam, ubar))) return rc;
   if ((rc = k_adj_solve<0>(ctx, us, x1, lam, lam, ubar))) return rc;
+  const Table& t = c.t_all;
+  hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us, lam,
+                     ubar, fbar, c.zd, t.d, c.K);
+  CK(hipGetLastError());
+  if (c.nz_ > 1 && c.t_face.n > 0) {
+    for (int xf = 1; xf >= 0; --xf) {
+      hipLaunchKernelGGL(fast::bt::adj_exch_kernel<kExchNT>, dim3(c.t_face.n), dim3(kExchNT), 0,
+                         c.stream, ubar, c.zd, c.faces, c.t_face.d, xf);
+      CK(hipGetLastError());
+    }
+  }
+  return NPBMZ_OK;
+}
+
+// ------------------------------------------------------------ SP adjoint
+// One batched SP step as separate launches, copying the post-exchange u, r0
+// = dt*compute_rhs, and the three solve outputs before their transforms when
+// the targets are given: compute_rhs, txinvr, then per direction the solve
+// without its transform, the transform, and finally add.
+int sp_step_store(npbmz_ctx* ctx, double* us, double* r0, double* x1, double* x2, double* x3) {
+  auto& c = *ctx;
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  const Table& ti = c.t_int;
+  int rc;
+  if ((rc = exch(ctx))) return rc;
+  if (us) CK(hipMemcpyAsync(us, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_rhs(ctx, -1, true))) return rc;
+  if (r0) CK(hipMemcpyAsync(r0, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  if (ti.n > 0)
+    hipLaunchKernelGGL(fast::sp::txinvr_kernel<kPtNT>, dim3(ti.n), dim3(kPtNT), 0, c.stream, c.u,
+                       c.rhs, c.zd, ti.d, c.K);
+  CK(hipGetLastError());
+  double* xs[3] = {x1, x2, x3};
+  for (int d = 0; d < 3; ++d) {
+    const Table& t = c.t_line[d];
+    int grid = t.n;
+    if (c.solve_grid[d] > 0) grid = std::min(t.n, c.solve_grid[d]);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(kLineNT), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
+                         t.d, t.n, c.maxrows[d], c.K);
+    };
+    if (d == 0) go(fast::sp::solve_kernel<0, false, false>);
+    if (d == 1) go(fast::sp::solve_kernel<1, false, false>);
+    if (d == 2) go(fast::sp::solve_kernel<2, false, false>);
+    CK(hipGetLastError());
+    if (xs[d]) CK(hipMemcpyAsync(xs[d], c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+    auto tr = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(ti.n), dim3(kPtNT), 0, c.stream, c.u, c.rhs, c.zd, ti.d, c.K);
+    };
+    if (ti.n > 0) {
+      if (d == 0) tr(fast::sp::transform_kernel<0, kPtNT>);
+      if (d == 1) tr(fast::sp::transform_kernel<1, kPtNT>);
+      if (d == 2) tr(fast::sp::transform_kernel<2, kPtNT>);
+    }
+    CK(hipGetLastError());
+  }
+  return k_add(ctx, -1);
+}
+
+template <int DIR>
+int k_sp_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double* gin,
+                   double* lam, double* ubar) {
+  auto& c = *ctx;
+  const Table& t = c.t_line[DIR];
+  if (t.n <= 0) return NPBMZ_OK;
+  int grid = t.n;
+  if (c.solve_grid[DIR] > 0) grid = std::min(t.n, c.solve_grid[DIR]);
+  hipLaunchKernelGGL(fast::sp::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us,
+                     x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  CK(hipGetLastError());
+  return NPBMZ_OK;
+}
+
+// Reverse one SP step from its stored forward data; ubar holds dL/du after
+// the step on entry and dL/du before it (before the exchange) on return;
+// fbar accumulates dL/dforcing; lam is workspace.
+int sp_adj_step(npbmz_ctx* ctx, const double* us, const double* r0, const double* x1,
+                const double* x2, const double* x3, double* ubar, double* lam, double* fbar) {
+  auto& c = *ctx;
+  int rc;
+  if ((rc = k_sp_adj_solve<2>(ctx, us, x3, ubar, lam, ubar))) return rc;
+  if ((rc = k_sp_adj_solve<1>(ctx, us, x2, lam, lam, ubar))) return rc;
+  if ((rc = k_sp_adj_solve<0>(ctx, us, x1, lam, lam, ubar))) return rc;
+  const Table& ti = c.t_int;
+  if (ti.n > 0)
+    hipLaunchKernelGGL(fast::sp::adj_txinvr_kernel<kPtNT>, dim3(ti.n), dim3(kPtNT), 0, c.stream,
+                       us, r0, lam, ubar, c.zd, ti.d, c.K);
+  CK(hipGetLastError());
   const Table& t = c.t_all;
   hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us, lam,
                      ubar, fbar, c.zd, t.d, c.K);
@@ -1344,7 +1433,8 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
                   double* adj_ms) {
   if (!ctx) return NPBMZ_EINVAL;
   auto& c = *ctx;
-  if (c.bench != NPBMZ_BT) return fail(ctx, NPBMZ_ENOTSUP, "the adjoint is built for BT");
+  if (c.bench == NPBMZ_LU) return fail(ctx, NPBMZ_ENOTSUP, "the adjoint is built for BT and SP");
+  const bool sp = c.bench == NPBMZ_SP;
   if (nsteps < 1 || ckpt_every < 1) return fail(ctx, NPBMZ_EINVAL, "nsteps and ckpt_every must be >= 1");
   if (!w || !grad_u0 || !grad_forcing || !loss) return fail(ctx, NPBMZ_EINVAL, "null argument");
   const size_t nd = 5 * (size_t)c.npoints, fb = sizeof(double) * nd;
@@ -1362,10 +1452,11 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
     for (double* p : bufs) (void)hipFree(p);
   };
   double *wd, *ubar, *lam, *fbar;
-  std::vector<double*> ck(ncp), us(k), x1(k), x2(k), x3(k);
+  std::vector<double*> ck(ncp), us(k), x1(k), x2(k), x3(k), r0(k, nullptr);
   bool ok = alloc(&wd) && alloc(&ubar) && alloc(&lam) && alloc(&fbar);
   for (int q = 0; q < ncp && ok; ++q) ok = alloc(&ck[q]);
   for (int q = 0; q < k && ok; ++q) ok = alloc(&us[q]) && alloc(&x1[q]) && alloc(&x2[q]) && alloc(&x3[q]);
+  for (int q = 0; q < k && ok && sp; ++q) ok = alloc(&r0[q]);
   if (!ok) {
     release();
     return fail(ctx, NPBMZ_ENOMEM, "adjoint buffers");
@@ -1381,7 +1472,9 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   if (rc == NPBMZ_OK) hip(hipEventRecord(e[0], c.stream), "record");
   for (int n = 0; n < nsteps && rc == NPBMZ_OK; ++n) {
     if (n % k == 0) hip(hipMemcpyAsync(ck[n / k], c.u, fb, hipMemcpyDeviceToDevice, c.stream), "checkpoint");
-    if (rc == NPBMZ_OK) rc = bt_step_store(ctx, nullptr, nullptr, nullptr, nullptr);
+    if (rc == NPBMZ_OK)
+      rc = sp ? sp_step_store(ctx, nullptr, nullptr, nullptr, nullptr, nullptr)
+              : bt_step_store(ctx, nullptr, nullptr, nullptr, nullptr);
   }
   if (rc == NPBMZ_OK) hip(hipEventRecord(e[1], c.stream), "record");
   if (rc == NPBMZ_OK) rc = bt_loss(ctx, wd, ubar, fbar, true, loss);
@@ -1391,10 +1484,16 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   for (int sgm = ncp - 1; sgm >= 0 && rc == NPBMZ_OK; --sgm) {
     const int n0 = sgm * k, n1 = std::min(nsteps, n0 + k);
     hip(hipMemcpyAsync(c.u, ck[sgm], fb, hipMemcpyDeviceToDevice, c.stream), "restore");
-    for (int n = n0; n < n1 && rc == NPBMZ_OK; ++n)
-      rc = bt_step_store(ctx, us[n - n0], x1[n - n0], x2[n - n0], x3[n - n0]);
-    for (int n = n1 - 1; n >= n0 && rc == NPBMZ_OK; --n)
-      rc = bt_adj_step(ctx, us[n - n0], x1[n - n0], x2[n - n0], x3[n - n0], ubar, lam, fbar);
+    for (int n = n0; n < n1 && rc == NPBMZ_OK; ++n) {
+      const int q = n - n0;
+      rc = sp ? sp_step_store(ctx, us[q], r0[q], x1[q], x2[q], x3[q])
+              : bt_step_store(ctx, us[q], x1[q], x2[q], x3[q]);
+    }
+    for (int n = n1 - 1; n >= n0 && rc == NPBMZ_OK; --n) {
+      const int q = n - n0;
+      rc = sp ? sp_adj_step(ctx, us[q], r0[q], x1[q], x2[q], x3[q], ubar, lam, fbar)
+              : bt_adj_step(ctx, us[q], x1[q], x2[q], x3[q], ubar, lam, fbar);
+    }
   }
   if (rc == NPBMZ_OK) hip(hipEventRecord(e[3], c.stream), "record");
   // results, and u back to the initial state

@@ -264,7 +264,7 @@ class Context:
         return out.value
 
     def adjoint(self, nsteps, ckpt_every, w):
-        """BT only: loss = sum(w*u_N**2) after nsteps steps from the current
+        """BT and SP: loss = sum(w*u_N**2) after nsteps steps from the current
         state, and its gradients w.r.t. the initial u and the forcing (both in
         get_all layout), with u kept every ckpt_every steps.  Returns a dict
         with loss, grad_u0, grad_forcing and the forward / reverse times in

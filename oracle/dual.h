@@ -92,11 +92,25 @@ inline Dual<N>& operator/=(Dual<N>& a, const Dual<N>& b) { return a = a / b; }
 template <int N>
 inline Dual<N>& operator*=(Dual<N>& a, double b) { return a = a * b; }
 template <int N>
+inline Dual<N>& operator+=(Dual<N>& a, double b) { return a = a + b; }
+template <int N>
+inline Dual<N>& operator-=(Dual<N>& a, double b) { return a = a - b; }
+template <int N>
+inline Dual<N>& operator/=(Dual<N>& a, double b) { return a = a / b; }
+template <int N>
 inline Dual<N> sqrt(const Dual<N>& a) {
   Dual<N> r;
   r.v = std::sqrt(a.v);
   for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / (2.0 * r.v);
   return r;
 }
+// max: the larger operand with its derivatives (the first one on a tie), the
+// derivative of max along the branch that attains it.
+template <int N>
+inline Dual<N> fmax(const Dual<N>& a, const Dual<N>& b) { return a.v >= b.v ? a : b; }
+template <int N>
+inline Dual<N> fmax(const Dual<N>& a, double b) { return a.v >= b ? a : Dual<N>(b); }
+template <int N>
+inline Dual<N> fmax(double a, const Dual<N>& b) { return a >= b.v ? Dual<N>(a) : b; }
 
 }  // namespace npb

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <vector>
 
+#include "dual.h"
 #include "zone_ops.h"
 
 namespace npb {
@@ -166,8 +167,8 @@ void assemble_line(Zone<T>& z, Lhs<T>& L, const Consts& K, int dir, int a, int b
     size_t d = z.didx(i, j, k);
     T ru1 = K.c3c4 * z.rho_i[d];
     cv[l] = dir == 0 ? z.us[d] : dir == 1 ? z.vs[d] : z.ws[d];
-    rhon[l] = std::fmax(std::fmax(std::fmax(dv + K.con43 * ru1, d5 + K.c1c5 * ru1), dmax + ru1),
-                        d1);
+    using std::fmax;
+    rhon[l] = fmax(fmax(fmax(dv + K.con43 * ru1, d5 + K.c1c5 * ru1), dmax + ru1), d1);
     spd[l] = z.speed[d];
   }
   for (int l = 1; l <= nl - 2; ++l) {
@@ -405,6 +406,72 @@ void oracle_sp_set(void* h, int z, int field, const double* in) {
 // for rows 0..nl-1, as assembled from the derived fields of the last
 // compute_rhs (rows 0 and nl-1 keep lhsinit's values).  Test hook; the
 // zone's own lhs is not modified.  out: nl x 15 row-major.
+// Forward-mode dual numbers through nsteps SP steps from the current state,
+// as oracle_bt_dual: out[0] = loss = sum(w*u_N^2), out[1+d] its derivative
+// along (vu[d] on u0, vf[d] on the forcing at interior points).  Arrays in
+// the whole-run layout; the state is left unchanged.
+void oracle_sp_dual(void* h, int nsteps, int ndir, const double* w, const double* vu,
+                    const double* vf, double* out) {
+  constexpr int ND = 4;
+  using D = Dual<ND>;
+  Run* r = (Run*)h;
+  const int nzn = r->G.count();
+  std::vector<size_t> zoff(nzn + 1, 0);
+  for (int z = 0; z < nzn; ++z)
+    zoff[z + 1] = zoff[z] + (size_t)5 * r->Z[z].nx * r->Z[z].ny * r->Z[z].nz;
+  for (int d0 = 0; d0 < std::max(ndir, 1); d0 += ND) {
+    std::vector<Zone<D>> Z(nzn);
+    std::vector<Lhs<D>> L(nzn);
+    for (int z = 0; z < nzn; ++z) {
+      Zone<double>& a = r->Z[z];
+      Zone<D>& b = Z[z];
+      b.nx = a.nx; b.ny = a.ny; b.nz = a.nz;
+      b.alloc();
+      L[z].alloc(b.nx, b.ny, b.nz);
+      lhsinit(L[z]);
+      for (size_t q = 0; q < a.u.size(); ++q) b.u[q] = D(a.u[q]);
+      for (size_t q = 0; q < a.forcing.size(); ++q) b.forcing[q] = D(a.forcing[q]);
+      size_t p = zoff[z];
+      for (int k = 0; k < a.nz; ++k)
+        for (int j = 0; j < a.ny; ++j)
+          for (int i = 0; i < a.nx; ++i)
+            for (int m = 0; m < 5; ++m, ++p)
+              for (int e = 0; e < ND && d0 + e < ndir; ++e) {
+                const size_t off = (size_t)(d0 + e) * zoff[nzn];
+                b.U(m, i, j, k).d[e] = vu[off + p];
+                const bool in = i > 0 && i < a.nx - 1 && j > 0 && j < a.ny - 1 && k > 0 && k < a.nz - 1;
+                if (in) b.F(m, i, j, k).d[e] = vf[off + p];
+              }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      if (nzn > 1) bt::exch_qbc(Z, r->G, true);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(r->nthreads)
+      for (int z = 0; z < nzn; ++z) {
+        bt::compute_rhs(Z[z], r->K, true);
+        txinvr(Z[z], r->K);
+        solve_dir(Z[z], L[z], r->K, 0);
+        solve_dir(Z[z], L[z], r->K, 1);
+        solve_dir(Z[z], L[z], r->K, 2);
+        bt::add(Z[z]);
+      }
+    }
+    D loss(0.0);
+    for (int z = 0; z < nzn; ++z) {
+      Zone<D>& b = Z[z];
+      size_t p = zoff[z];
+      for (int k = 0; k < b.nz; ++k)
+        for (int j = 0; j < b.ny; ++j)
+          for (int i = 0; i < b.nx; ++i)
+            for (int m = 0; m < 5; ++m, ++p) {
+              const D& x = b.U(m, i, j, k);
+              loss = loss + w[p] * (x * x);
+            }
+    }
+    out[0] = loss.v;
+    for (int e = 0; e < ND && d0 + e < ndir; ++e) out[1 + d0 + e] = loss.d[e];
+  }
+}
+
 void oracle_sp_line_lhs(void* h, int z, int dir, int a, int b, double* out) {
   Run* r = (Run*)h;
   Lhs<double> L = r->L[z];

@@ -61,8 +61,9 @@ def load():
         getattr(L, f"oracle_lu_{n}").argtypes = a
         getattr(L, f"oracle_lu_{n}").restype = None
     L.oracle_lu_nzones.argtypes = [vp]
-    L.oracle_bt_dual.argtypes = [vp, i, i, dp, dp, dp, dp]
-    L.oracle_bt_dual.restype = None
+    for b in ("bt", "sp"):
+        getattr(L, f"oracle_{b}_dual").argtypes = [vp, i, i, dp, dp, dp, dp]
+        getattr(L, f"oracle_{b}_dual").restype = None
     _lib = L
     return L
 
@@ -132,11 +133,8 @@ class _OracleMZ:
         return xcr, xce
 
 
-class OracleBT(_OracleMZ):
-    PREFIX = "bt"
-
     def dual_gradient(self, nsteps, w, vu, vf):
-        """Forward-mode dual numbers through nsteps BT steps from the current
+        """Forward-mode dual numbers through nsteps steps from the current
         state: returns (loss, d) with loss = sum(w*u_N**2) and d[i] the
         derivative of the loss along (vu[i] on u0, vf[i] on the forcing).
         Arrays in the whole-run layout (every zone's [k][j][i][m], zone after
@@ -146,8 +144,12 @@ class OracleBT(_OracleMZ):
         w = np.ascontiguousarray(w, dtype=np.float64)
         nd = vu.shape[0]
         out = np.zeros(1 + nd)
-        self.L.oracle_bt_dual(self.h, int(nsteps), nd, _dp(w), _dp(vu), _dp(vf), _dp(out))
+        self._f("dual")(self.h, int(nsteps), nd, _dp(w), _dp(vu), _dp(vf), _dp(out))
         return out[0], out[1:]
+
+
+class OracleBT(_OracleMZ):
+    PREFIX = "bt"
 
     def line_blocks(self, z, d, a, b):
         nl = self.dims(z)[d]

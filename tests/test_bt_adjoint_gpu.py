@@ -133,9 +133,9 @@ def test_adjoint_matches_dual_numbers_class_A(gpu):
     ctx.close()
 
 
-def test_adjoint_is_bt_only(gpu):
-    zones, p = gpu.zones_for("SP", "S")
-    ctx = gpu.Context("SP", zones, p.dt)
-    with pytest.raises(gpu.NpbmzError, match="BT"):
+def test_adjoint_is_not_built_for_lu(gpu):
+    zones, p = gpu.zones_for("LU", "S")
+    ctx = gpu.Context("LU", zones, p.dt, omega=1.2)
+    with pytest.raises(gpu.NpbmzError, match="BT and SP"):
         ctx.adjoint(2, 1, np.zeros(ctx.size))
     ctx.close()

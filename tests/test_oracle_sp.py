@@ -99,3 +99,43 @@ def test_sp_compute_rhs_equals_bt_compute_rhs():
     assert all(np.array_equal(a, c) for a, c in zip(s.norms(), b.norms()))
     s.close()
     b.close()
+
+
+def test_dual_numbers_match_central_differences():
+    """The SP oracle's forward-mode dual run (the SP adjoint's reference
+    gradient) against central differences of its own double-precision run:
+    class S, 5 steps, two directions, the loss difference summed point by
+    point at h = 1e-4.  rhon's max is differentiated along the branch that
+    attains it; the seeded directions cross no tie."""
+    zones, p = P.zones_for("SP", "S")
+    o = OracleSP.for_class(p)
+    r = np.random.Generator(np.random.Philox(12))
+    shapes = [(zones.nz[z], zones.ny[z], zones.nx[z], 5) for z in range(zones.count)]
+    n = sum(int(np.prod(s)) for s in shapes)
+    w = r.random(n)
+    mask = np.concatenate([np.pad(np.ones((a - 2, b - 2, c - 2, 5)),
+                                  ((1, 1), (1, 1), (1, 1), (0, 0))).ravel() for a, b, c, _ in shapes])
+    vu = r.standard_normal((2, n))
+    vf = r.standard_normal((2, n)) * mask
+    L, d = o.dual_gradient(5, w, vu, vf)
+    u0 = [o.get(z, 0) for z in range(zones.count)]
+    f0 = [o.get(z, 1) for z in range(zones.count)]
+
+    def final(du, df):
+        off = 0
+        for z, s in enumerate(shapes):
+            k = int(np.prod(s))
+            o.set(z, 0, u0[z] + du[off:off + k].reshape(s))
+            o.set(z, 1, f0[z] + df[off:off + k].reshape(s))
+            off += k
+        o.step(5)
+        return np.concatenate([o.get(z, 0).ravel() for z in range(zones.count)])
+
+    base = final(0 * vu[0], 0 * vf[0])
+    assert abs(L / np.sum(w * base * base) - 1) < 1e-13
+    h = 1e-4
+    for q in range(2):
+        up, um = final(h * vu[q], h * vf[q]), final(-h * vu[q], -h * vf[q])
+        fd = np.sum(w * (up - um) * (up + um)) / (2 * h)
+        assert abs(fd / d[q] - 1) < 1e-7, (fd, d[q])
+    o.close()
