@@ -633,6 +633,8 @@ int bt_step_store(npbmz_ctx* ctx, double* us, double* x1, double* x2, double* x3
   return k_add(ctx, -1);
 }
 
+// The transposed solve of direction DIR (lam = A^-T gin), then the
+// derivative of its blocks into ubar.
 template <int DIR>
 int k_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double* gin, double* lam,
                 double* ubar) {
@@ -641,8 +643,13 @@ int k_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double*
   if (t.n <= 0) return NPBMZ_OK;
   int grid = t.n;
   if (c.solve_grid[DIR] > 0) grid = std::min(t.n, c.solve_grid[DIR]);
-  hipLaunchKernelGGL(fast::bt::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us, x,
-                     gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  hipLaunchKernelGGL(fast::bt::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us,
+                     gin, lam, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  CK(hipGetLastError());
+  const Table& ti = c.t_int;
+  if (ti.n > 0)
+    hipLaunchKernelGGL((fast::bt::adj_jac_kernel<DIR, kPtNT>), dim3(ti.n), dim3(kPtNT), 0,
+                       c.stream, us, x, lam, ubar, c.zd, ti.d, c.K);
   CK(hipGetLastError());
   return NPBMZ_OK;
 }
