@@ -95,6 +95,24 @@ def sp_adjoint_case(P):
             "ratio": round(r["adj_s"] / r["fwd_s"], 3)}
 
 
+def lu_adjoint_case(P):
+    """The LU adjoint on class A (128x128x16, 4x4 equal zones, NPB's exchange
+    planes), 100 iterations, checkpoints every 20, loss sum(w u^2) with w
+    uniform [0,1) (Philox seed 20)."""
+    import numpy as np
+    zones, p = P.zones_for("LU", "A")
+    ctx = P.Context("LU", zones, p.dt, omega=1.2, npb_exchange=True)
+    ctx.init_exact()
+    w = np.random.Generator(np.random.Philox(20)).random(ctx.size)
+    ctx.adjoint(100, 20, w)  # warm-up
+    r = ctx.adjoint(100, 20, w)
+    ctx.close()
+    return {"workload": "LU-MZ class A, 4x4 zones, 100 iterations, checkpoint every 20",
+            "forward_ms_per_step": round(r["fwd_s"] / 100 * 1e3, 5),
+            "adjoint_ms_per_step": round(r["adj_s"] / 100 * 1e3, 5),
+            "ratio": round(r["adj_s"] / r["fwd_s"], 3)}
+
+
 def class_run(P, bench, cls, npb_exchange=False):
     """A whole class run (graph mode) and the same steps launched per zone."""
     zones, p = P.zones_for(bench, cls)
@@ -188,6 +206,7 @@ def main():
     if not args.no_extra:
         extra["adjoint"] = adjoint_case(P)
         extra["sp_adjoint"] = sp_adjoint_case(P)
+        extra["lu_adjoint"] = lu_adjoint_case(P)
         extra["sp_class_A"] = class_run(P, "SP", "A")
         extra["lu_class_A"] = class_run(P, "LU", "A", npb_exchange=True)
 
@@ -219,6 +238,7 @@ def main():
         "adjoint_forward_ratio": extra.get("adjoint", {}).get("ratio"),
         "adjoint": extra.get("adjoint"),
         "sp_adjoint": extra.get("sp_adjoint"),
+        "lu_adjoint": extra.get("lu_adjoint"),
         "sp_class_A": extra.get("sp_class_A"),
         "lu_class_A": extra.get("lu_class_A"),
         "phase_ms_per_step": {n: round(per_launch_s[n] * 1e3, 5) for n in PHASES},

@@ -183,21 +183,27 @@ int npbmz_describe(const npbmz_ctx* ctx, char* buf, int32_t cap);
 /* Number of zones and total grid points held by ctx. */
 int npbmz_size(const npbmz_ctx* ctx, int32_t* nzones, int64_t* npoints);
 
-/* Adjoint of the BT and SP timesteps (no counterpart in the reference, which
- * has no derivatives; a ChainRulesCore rrule's pullback would call it).  From
- * the context's current u (u0) and forcing, runs nsteps timesteps keeping u
- * every ckpt_every steps, evaluates loss = sum over every grid point and
- * component of w*u^2 on the final u, then runs the steps backwards (each
- * segment between checkpoints recomputed forward first): the transposed
- * z, y and x solves (BT block-tridiagonal; SP pentadiagonal, after the
- * adjoints of tzetar, pinvr and ninvr) with the derivatives of their
- * matrices, the adjoint of txinvr (SP), of compute_rhs and of the face
- * exchange.  grad_u0 and grad_forcing receive
- * dloss/du0 and dloss/dforcing; fwd_ms and adj_ms (may be NULL) the
- * HIP-event times of the forward pass and of the reverse pass including its
- * recomputation.  w and both gradients hold every zone's nx*ny*nz*5 values,
- * zone after zone, each zone in [k][j][i][m] order (m fastest).  On return u
- * is u0 again.  NPBMZ_ENOTSUP for LU. */
+/* Adjoint of the BT, SP and LU timesteps (no counterpart in the reference,
+ * which has no derivatives; a ChainRulesCore rrule's pullback would call it).
+ * From the context's current u (u0) and forcing (LU: frct), runs nsteps
+ * timesteps keeping u every ckpt_every steps, evaluates loss = sum over every
+ * grid point and component of w*u^2 on the final u, then runs the steps
+ * backwards (each segment between checkpoints recomputed forward first):
+ * BT and SP: the transposed z, y and x solves (BT block-tridiagonal; SP
+ * pentadiagonal, after the adjoints of tzetar, pinvr and ninvr) with the
+ * derivatives of their matrices, the adjoint of txinvr (SP), of compute_rhs
+ * and of the face exchange; LU: the transposed upper and lower SSOR sweeps
+ * (hyperplanes in the opposite order) with the derivatives of jacld's and
+ * jacu's blocks, the adjoint of the exchange and of rhs.  For LU, rsd is
+ * first recomputed as rhs(u0), the value an iteration starts from, so the
+ * loss is a function of u0 and frct alone, and frct's gradient covers every
+ * point (rsd = -frct on the boundary enters the sweeps).  grad_u0 and
+ * grad_forcing receive dloss/du0 and dloss/dforcing; fwd_ms and adj_ms (may
+ * be NULL) the HIP-event times of the forward pass and of the reverse pass
+ * including its recomputation.  w and both gradients hold every zone's
+ * nx*ny*nz*5 values, zone after zone, each zone in [k][j][i][m] order (m
+ * fastest).  On return u (LU: and rsd) hold their values from before the
+ * call. */
 int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const double* w,
                   double* grad_u0, double* grad_forcing, double* loss, double* fwd_ms,
                   double* adj_ms);

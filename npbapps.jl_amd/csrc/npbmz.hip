@@ -112,6 +112,7 @@ __device__ __forceinline__ double recip(double x) {
 #include "sp_kernels.inc"
 #include "sp_adjoint.inc"
 #include "lu_kernels.inc"
+#include "lu_adjoint.inc"
 }  // namespace fast
 
 }  // namespace npbmz
@@ -763,6 +764,61 @@ int sp_adj_step(npbmz_ctx* ctx, const double* us, const double* r0, const double
       CK(hipGetLastError());
     }
   }
+  return NPBMZ_OK;
+}
+
+// ------------------------------------------------------------ LU adjoint
+// One batched LU iteration (exch_qbc, lower and upper sweeps, update, rhs),
+// copying u before and after the exchange and the outputs of the two sweeps
+// when the targets are given.
+int lu_step_store(npbmz_ctx* ctx, double* un, double* ue, double* r1, double* r2) {
+  auto& c = *ctx;
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  int rc;
+  if (un) CK(hipMemcpyAsync(un, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = exch(ctx))) return rc;
+  if (ue) CK(hipMemcpyAsync(ue, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_lu_sweep(ctx, false, -1))) return rc;
+  if (r1) CK(hipMemcpyAsync(r1, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_lu_sweep(ctx, true, -1))) return rc;
+  if (r2) CK(hipMemcpyAsync(r2, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
+  if ((rc = k_lu_update(ctx, -1))) return rc;
+  return k_lu_rhs(ctx, -1);
+}
+
+constexpr int kAdjSweepNT = 256;  // threads of a transposed-sweep workgroup (one per zone)
+
+// Reverse one LU iteration from its stored forward data (un, ue: u before
+// and after the exchange; r1, r2: the sweeps' outputs): ubar holds dL/du
+// after the iteration on entry and dL/du before it on return; fbar
+// accumulates dL/dfrct; nu and rho are workspace.
+int lu_adj_step(npbmz_ctx* ctx, const double* un, const double* ue, const double* r1,
+                const double* r2, double* ubar, double* nu, double* rho, double* fbar) {
+  auto& c = *ctx;
+  const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
+  CK(hipMemsetAsync(nu, 0, fb, c.stream));
+  CK(hipMemsetAsync(rho, 0, fb, c.stream));
+  hipLaunchKernelGGL((fast::lu::adj_sweep_zone_kernel<true, kAdjSweepNT>), dim3(c.nz_),
+                     dim3(kAdjSweepNT), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
+  CK(hipGetLastError());
+  hipLaunchKernelGGL((fast::lu::adj_sweep_zone_kernel<false, kAdjSweepNT>), dim3(c.nz_),
+                     dim3(kAdjSweepNT), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
+  CK(hipGetLastError());
+  const Table& t = c.t_all;
+  hipLaunchKernelGGL(fast::lu::adj_blocks_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, ue,
+                     r1, r2, nu, rho, ubar, fbar, c.zd, t.d, c.lk);
+  CK(hipGetLastError());
+  // the exchange's two launches in reverse order
+  const Table* ts[2] = {&c.t_face2, &c.t_face};
+  for (const Table* tf : ts) {
+    if (tf->n <= 0) continue;
+    hipLaunchKernelGGL(fast::bt::adj_exch_kernel<kExchNT>, dim3(tf->n), dim3(kExchNT), 0, c.stream,
+                       ubar, c.zd, c.faces, tf->d, -1);
+    CK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(fast::lu::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, un, rho,
+                     ubar, c.zd, t.d, c.lk);
+  CK(hipGetLastError());
   return NPBMZ_OK;
 }
 
@@ -1440,15 +1496,16 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
                   double* adj_ms) {
   if (!ctx) return NPBMZ_EINVAL;
   auto& c = *ctx;
-  if (c.bench == NPBMZ_LU) return fail(ctx, NPBMZ_ENOTSUP, "the adjoint is built for BT and SP");
-  const bool sp = c.bench == NPBMZ_SP;
+  const bool sp = c.bench == NPBMZ_SP, lu = c.bench == NPBMZ_LU;
   if (nsteps < 1 || ckpt_every < 1) return fail(ctx, NPBMZ_EINVAL, "nsteps and ckpt_every must be >= 1");
   if (!w || !grad_u0 || !grad_forcing || !loss) return fail(ctx, NPBMZ_EINVAL, "null argument");
   const size_t nd = 5 * (size_t)c.npoints, fb = sizeof(double) * nd;
   const int k = std::min(ckpt_every, nsteps);
   const int ncp = (nsteps + k - 1) / k;
   // device buffers: w, ubar, lam, fbar, the checkpoints, and per step of a
-  // segment the post-exchange u and the three solve outputs
+  // segment the post-exchange u and the three solve outputs (SP: also r0;
+  // LU: the pre-exchange u in x3 and the sweep outputs in x1, x2, with one
+  // more workspace field)
   std::vector<double*> bufs;
   auto alloc = [&](double** p) -> bool {
     if (hipMalloc(p, fb) != hipSuccess) return false;
@@ -1458,9 +1515,10 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   auto release = [&]() {
     for (double* p : bufs) (void)hipFree(p);
   };
-  double *wd, *ubar, *lam, *fbar;
+  double *wd, *ubar, *lam, *fbar, *aux = nullptr, *rs0 = nullptr;
   std::vector<double*> ck(ncp), us(k), x1(k), x2(k), x3(k), r0(k, nullptr);
-  bool ok = alloc(&wd) && alloc(&ubar) && alloc(&lam) && alloc(&fbar);
+  bool ok = alloc(&wd) && alloc(&ubar) && alloc(&lam) && alloc(&fbar) &&
+            (!lu || (alloc(&aux) && alloc(&rs0)));
   for (int q = 0; q < ncp && ok; ++q) ok = alloc(&ck[q]);
   for (int q = 0; q < k && ok; ++q) ok = alloc(&us[q]) && alloc(&x1[q]) && alloc(&x2[q]) && alloc(&x3[q]);
   for (int q = 0; q < k && ok && sp; ++q) ok = alloc(&r0[q]);
@@ -1473,6 +1531,11 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
     if (e != hipSuccess && rc == NPBMZ_OK) rc = fail(ctx, NPBMZ_EHIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hip(hipMemcpyAsync(wd, w, fb, hipMemcpyHostToDevice, c.stream), "weights upload");
+  // LU's rsd at an iteration's start is rhs(u): recomputed, so that the loss
+  // is a function of u0 and frct alone (the context's own rsd is put back at
+  // the end)
+  if (lu) hip(hipMemcpyAsync(rs0, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream), "save rsd");
+  if (lu && rc == NPBMZ_OK) rc = k_lu_rhs(ctx, -1);
   hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   for (auto& x : e) hip(hipEventCreate(&x), "hipEventCreate");
   // forward pass, keeping u every k steps
@@ -1480,8 +1543,9 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   for (int n = 0; n < nsteps && rc == NPBMZ_OK; ++n) {
     if (n % k == 0) hip(hipMemcpyAsync(ck[n / k], c.u, fb, hipMemcpyDeviceToDevice, c.stream), "checkpoint");
     if (rc == NPBMZ_OK)
-      rc = sp ? sp_step_store(ctx, nullptr, nullptr, nullptr, nullptr, nullptr)
-              : bt_step_store(ctx, nullptr, nullptr, nullptr, nullptr);
+      rc = sp   ? sp_step_store(ctx, nullptr, nullptr, nullptr, nullptr, nullptr)
+           : lu ? lu_step_store(ctx, nullptr, nullptr, nullptr, nullptr)
+                : bt_step_store(ctx, nullptr, nullptr, nullptr, nullptr);
   }
   if (rc == NPBMZ_OK) hip(hipEventRecord(e[1], c.stream), "record");
   if (rc == NPBMZ_OK) rc = bt_loss(ctx, wd, ubar, fbar, true, loss);
@@ -1491,15 +1555,18 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   for (int sgm = ncp - 1; sgm >= 0 && rc == NPBMZ_OK; --sgm) {
     const int n0 = sgm * k, n1 = std::min(nsteps, n0 + k);
     hip(hipMemcpyAsync(c.u, ck[sgm], fb, hipMemcpyDeviceToDevice, c.stream), "restore");
+    if (lu && rc == NPBMZ_OK) rc = k_lu_rhs(ctx, -1);
     for (int n = n0; n < n1 && rc == NPBMZ_OK; ++n) {
       const int q = n - n0;
-      rc = sp ? sp_step_store(ctx, us[q], r0[q], x1[q], x2[q], x3[q])
-              : bt_step_store(ctx, us[q], x1[q], x2[q], x3[q]);
+      rc = sp   ? sp_step_store(ctx, us[q], r0[q], x1[q], x2[q], x3[q])
+           : lu ? lu_step_store(ctx, x3[q], us[q], x1[q], x2[q])
+                : bt_step_store(ctx, us[q], x1[q], x2[q], x3[q]);
     }
     for (int n = n1 - 1; n >= n0 && rc == NPBMZ_OK; --n) {
       const int q = n - n0;
-      rc = sp ? sp_adj_step(ctx, us[q], r0[q], x1[q], x2[q], x3[q], ubar, lam, fbar)
-              : bt_adj_step(ctx, us[q], x1[q], x2[q], x3[q], ubar, lam, fbar);
+      rc = sp   ? sp_adj_step(ctx, us[q], r0[q], x1[q], x2[q], x3[q], ubar, lam, fbar)
+           : lu ? lu_adj_step(ctx, x3[q], us[q], x1[q], x2[q], ubar, lam, aux, fbar)
+                : bt_adj_step(ctx, us[q], x1[q], x2[q], x3[q], ubar, lam, fbar);
     }
   }
   if (rc == NPBMZ_OK) hip(hipEventRecord(e[3], c.stream), "record");
@@ -1508,6 +1575,7 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
     hip(hipMemcpyAsync(grad_u0, ubar, fb, hipMemcpyDeviceToHost, c.stream), "gradient download");
     hip(hipMemcpyAsync(grad_forcing, fbar, fb, hipMemcpyDeviceToHost, c.stream), "gradient download");
     hip(hipMemcpyAsync(c.u, ck[0], fb, hipMemcpyDeviceToDevice, c.stream), "restore u0");
+    if (lu) hip(hipMemcpyAsync(c.rhs, rs0, fb, hipMemcpyDeviceToDevice, c.stream), "restore rsd");
     hip(hipStreamSynchronize(c.stream), "synchronize");
   }
   if (rc == NPBMZ_OK) {

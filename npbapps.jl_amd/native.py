@@ -264,9 +264,10 @@ class Context:
         return out.value
 
     def adjoint(self, nsteps, ckpt_every, w):
-        """BT and SP: loss = sum(w*u_N**2) after nsteps steps from the current
-        state, and its gradients w.r.t. the initial u and the forcing (both in
-        get_all layout), with u kept every ckpt_every steps.  Returns a dict
+        """loss = sum(w*u_N**2) after nsteps steps from the current state, and
+        its gradients w.r.t. the initial u and the forcing (LU: frct, whose
+        rsd = rhs(u0) is recomputed first), both in get_all layout, with u
+        kept every ckpt_every steps.  Returns a dict
         with loss, grad_u0, grad_forcing and the forward / reverse times in
         seconds; u is left at its initial value."""
         w = np.ascontiguousarray(w, dtype=np.float64)

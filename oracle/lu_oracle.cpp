@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 
+#include "dual.h"
 #include "npb_common.h"
 
 namespace npb {
@@ -42,7 +43,8 @@ static const double dz1 = 1.00, dz2 = 1.00, dz3 = 1.00, dz4 = 1.00, dz5 = 1.00;
 static const double dssp = std::max(std::max(dx1, dy1), dz1) / 4.0;
 static const int ii1 = 2, ji1 = 2, ki1 = 3;
 
-static inline double sq(double x) { return x * x; }
+template <class T>
+static inline T sq(const T& x) { return x * x; }
 
 // setcoeff, src/LU/setcoeff.jl:43-70.  The reference keeps these in module
 // globals set per zone in the setup loop, so every later call sees the last
@@ -72,39 +74,45 @@ static Coeff setcoeff(int nx0, int ny0, int nz0) {
   return c;
 }
 
-// A 5 x nx x ny x nz field, 1-based.
-struct Field {
+// A 5 x nx x ny x nz field, 1-based, of scalars T (double, or the dual
+// numbers of oracle_lu_dual).
+template <class T>
+struct FieldT {
   int nx = 0, ny = 0, nz = 0;
-  std::vector<double> v;
+  std::vector<T> v;
   void alloc(int a, int b, int c) {
     nx = a; ny = b; nz = c;
-    v.assign((size_t)5 * a * b * c, 0.0);
+    v.assign((size_t)5 * a * b * c, T(0.0));
   }
-  double& operator()(int m, int i, int j, int k) {
+  T& operator()(int m, int i, int j, int k) {
     return v[(size_t)(m - 1) + 5 * ((size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k - 1)))];
   }
-  double* at(int i, int j, int k) { return &(*this)(1, i, j, k); }
+  T* at(int i, int j, int k) { return &(*this)(1, i, j, k); }
 };
+using Field = FieldT<double>;
 
 // Per-row 5 x 5 x nx block workspace of jacld/jacu, src/LU/lu_data.jl:89-92.
-struct Blocks {
-  std::vector<double> v;
-  void alloc(int nx) { v.assign((size_t)25 * (nx + 2), 0.0); }
-  double& operator()(int m, int n, int i) { return v[(size_t)(m - 1) + 5 * ((n - 1) + 5 * (size_t)i)]; }
+template <class T>
+struct BlocksT {
+  std::vector<T> v;
+  void alloc(int nx) { v.assign((size_t)25 * (nx + 2), T(0.0)); }
+  T& operator()(int m, int n, int i) { return v[(size_t)(m - 1) + 5 * ((n - 1) + 5 * (size_t)i)]; }
 };
 
-struct Zone {
+template <class T>
+struct ZoneT {
   int nx, ny, nz;
-  Field u, rsd, frct, flux;
-  Blocks a, b, c, d;
-  std::vector<double> tv;  // 5 x nx, src/LU/ssor.jl:49
+  FieldT<T> u, rsd, frct, flux;
+  BlocksT<T> a, b, c, d;
+  std::vector<T> tv;  // 5 x nx, src/LU/ssor.jl:49
   void alloc() {
     u.alloc(nx, ny, nz); rsd.alloc(nx, ny, nz); frct.alloc(nx, ny, nz); flux.alloc(nx, ny, nz);
     a.alloc(nx); b.alloc(nx); c.alloc(nx); d.alloc(nx);
-    tv.assign((size_t)5 * (nx + 2), 0.0);
+    tv.assign((size_t)5 * (nx + 2), T(0.0));
   }
-  double& TV(int m, int i) { return tv[(size_t)(m - 1) + 5 * (size_t)i]; }
+  T& TV(int m, int i) { return tv[(size_t)(m - 1) + 5 * (size_t)i]; }
 };
+using Zone = ZoneT<double>;
 
 // exact, src/LU/exact.jl:7-22 (nested products; the same values as BT's
 // exact_solution, whose products only have their operands swapped).
@@ -419,13 +427,14 @@ static void erhs(Zone& z, const Coeff& C) {
 
 // rhs, src/LU/rhs.jl:5-519: rsd = -frct everywhere, then the flux
 // differences, viscous terms and fourth-order dissipation on the interior.
-static void rhs(Zone& z, const Coeff& C) {
+template <class T>
+static void rhs(ZoneT<T>& z, const Coeff& C) {
   const int nx = z.nx, ny = z.ny, nz = z.nz;
   const int ist = 2, iend = nx - 1, jst = 2, jend = ny - 1;
-  Field& u = z.u;
-  Field& rsd = z.rsd;
-  Field& frct = z.frct;
-  Field& flux = z.flux;
+  FieldT<T>& u = z.u;
+  FieldT<T>& rsd = z.rsd;
+  FieldT<T>& frct = z.frct;
+  FieldT<T>& flux = z.flux;
   const double tx1 = C.tx1, tx2 = C.tx2, tx3 = C.tx3, ty1 = C.ty1, ty2 = C.ty2, ty3 = C.ty3;
   const double tz1 = C.tz1, tz2 = C.tz2, tz3 = C.tz3;
   for (size_t p = 0; p < rsd.v.size(); ++p) rsd.v[p] = -frct.v[p];
@@ -435,8 +444,8 @@ static void rhs(Zone& z, const Coeff& C) {
     for (int j = jst; j <= jend; ++j) {
       for (int i = L1; i <= L2; ++i) {
         flux(1, i, j, k) = u(2, i, j, k);
-        double u21 = u(2, i, j, k) / u(1, i, j, k);
-        double q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
+        T u21 = u(2, i, j, k) / u(1, i, j, k);
+        T q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
                                u(4, i, j, k) * u(4, i, j, k)) /
                    u(1, i, j, k);
         flux(2, i, j, k) = u(2, i, j, k) * u21 + c2 * (u(5, i, j, k) - q);
@@ -448,12 +457,12 @@ static void rhs(Zone& z, const Coeff& C) {
         for (int m = 1; m <= 5; ++m)
           rsd(m, i, j, k) -= tx2 * (flux(m, i + 1, j, k) - flux(m, i - 1, j, k));
       for (int i = ist; i <= L2; ++i) {
-        double tmp = 1.0e+00 / u(1, i, j, k);
-        double u21i = tmp * u(2, i, j, k), u31i = tmp * u(3, i, j, k);
-        double u41i = tmp * u(4, i, j, k), u51i = tmp * u(5, i, j, k);
+        T tmp = 1.0e+00 / u(1, i, j, k);
+        T u21i = tmp * u(2, i, j, k), u31i = tmp * u(3, i, j, k);
+        T u41i = tmp * u(4, i, j, k), u51i = tmp * u(5, i, j, k);
         tmp = 1.0e+00 / u(1, i - 1, j, k);
-        double u21im1 = tmp * u(2, i - 1, j, k), u31im1 = tmp * u(3, i - 1, j, k);
-        double u41im1 = tmp * u(4, i - 1, j, k), u51im1 = tmp * u(5, i - 1, j, k);
+        T u21im1 = tmp * u(2, i - 1, j, k), u31im1 = tmp * u(3, i - 1, j, k);
+        T u41im1 = tmp * u(4, i - 1, j, k), u51im1 = tmp * u(5, i - 1, j, k);
         flux(2, i, j, k) = (4.0e+00 / 3.0e+00) * tx3 * (u21i - u21im1);
         flux(3, i, j, k) = tx3 * (u31i - u31im1);
         flux(4, i, j, k) = tx3 * (u41i - u41im1);
@@ -495,8 +504,8 @@ static void rhs(Zone& z, const Coeff& C) {
     for (int j = M1; j <= M2; ++j)
       for (int i = ist; i <= iend; ++i) {
         flux(1, i, j, k) = u(3, i, j, k);
-        double u31 = u(3, i, j, k) / u(1, i, j, k);
-        double q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
+        T u31 = u(3, i, j, k) / u(1, i, j, k);
+        T q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
                                u(4, i, j, k) * u(4, i, j, k)) /
                    u(1, i, j, k);
         flux(2, i, j, k) = u(2, i, j, k) * u31;
@@ -510,12 +519,12 @@ static void rhs(Zone& z, const Coeff& C) {
           rsd(m, i, j, k) -= ty2 * (flux(m, i, j + 1, k) - flux(m, i, j - 1, k));
     for (int j = jst; j <= M2; ++j)
       for (int i = ist; i <= iend; ++i) {
-        double tmp = 1.0e+00 / u(1, i, j, k);
-        double u21j = tmp * u(2, i, j, k), u31j = tmp * u(3, i, j, k);
-        double u41j = tmp * u(4, i, j, k), u51j = tmp * u(5, i, j, k);
+        T tmp = 1.0e+00 / u(1, i, j, k);
+        T u21j = tmp * u(2, i, j, k), u31j = tmp * u(3, i, j, k);
+        T u41j = tmp * u(4, i, j, k), u51j = tmp * u(5, i, j, k);
         tmp = 1.0e+00 / u(1, i, j - 1, k);
-        double u21jm1 = tmp * u(2, i, j - 1, k), u31jm1 = tmp * u(3, i, j - 1, k);
-        double u41jm1 = tmp * u(4, i, j - 1, k), u51jm1 = tmp * u(5, i, j - 1, k);
+        T u21jm1 = tmp * u(2, i, j - 1, k), u31jm1 = tmp * u(3, i, j - 1, k);
+        T u41jm1 = tmp * u(4, i, j - 1, k), u51jm1 = tmp * u(5, i, j - 1, k);
         flux(2, i, j, k) = ty3 * (u21j - u21jm1);
         flux(3, i, j, k) = (4.0e+00 / 3.0e+00) * ty3 * (u31j - u31jm1);
         flux(4, i, j, k) = ty3 * (u41j - u41jm1);
@@ -560,8 +569,8 @@ static void rhs(Zone& z, const Coeff& C) {
     for (int j = jst; j <= jend; ++j)
       for (int i = ist; i <= iend; ++i) {
         flux(1, i, j, k) = u(4, i, j, k);
-        double u41 = u(4, i, j, k) / u(1, i, j, k);
-        double q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
+        T u41 = u(4, i, j, k) / u(1, i, j, k);
+        T q = 0.50e+00 * (u(2, i, j, k) * u(2, i, j, k) + u(3, i, j, k) * u(3, i, j, k) +
                                u(4, i, j, k) * u(4, i, j, k)) /
                    u(1, i, j, k);
         flux(2, i, j, k) = u(2, i, j, k) * u41;
@@ -577,12 +586,12 @@ static void rhs(Zone& z, const Coeff& C) {
   for (int k = 2; k <= nz; ++k)
     for (int j = jst; j <= jend; ++j)
       for (int i = ist; i <= iend; ++i) {
-        double tmp = 1.0e+00 / u(1, i, j, k);
-        double u21k = tmp * u(2, i, j, k), u31k = tmp * u(3, i, j, k);
-        double u41k = tmp * u(4, i, j, k), u51k = tmp * u(5, i, j, k);
+        T tmp = 1.0e+00 / u(1, i, j, k);
+        T u21k = tmp * u(2, i, j, k), u31k = tmp * u(3, i, j, k);
+        T u41k = tmp * u(4, i, j, k), u51k = tmp * u(5, i, j, k);
         tmp = 1.0e+00 / u(1, i, j, k - 1);
-        double u21km1 = tmp * u(2, i, j, k - 1), u31km1 = tmp * u(3, i, j, k - 1);
-        double u41km1 = tmp * u(4, i, j, k - 1), u51km1 = tmp * u(5, i, j, k - 1);
+        T u21km1 = tmp * u(2, i, j, k - 1), u31km1 = tmp * u(3, i, j, k - 1);
+        T u41km1 = tmp * u(4, i, j, k - 1), u51km1 = tmp * u(5, i, j, k - 1);
         flux(2, i, j, k) = tz3 * (u21k - u21km1);
         flux(3, i, j, k) = tz3 * (u31k - u31km1);
         flux(4, i, j, k) = (4.0e+00 / 3.0e+00) * tz3 * (u41k - u41km1);
@@ -648,12 +657,13 @@ static Jac make_jac(const Coeff& C, double dt) {
 }
 
 // d block at the point: src/LU/jacld.jl:31-123 (src/LU/jacu.jl the same).
-static void block_d(const Jac& J, const double* uu, double D[5][5]) {
+template <class T>
+static void block_d(const Jac& J, const T* uu, T D[5][5]) {
   const double dt = J.dt, tx1 = J.tx1, ty1 = J.ty1, tz1 = J.tz1, r43 = J.r43, c1345 = J.c1345, c34 = J.c34;
-  const double u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
-  double tmp1 = 1.0e+00 / uu[0];
-  double tmp2 = tmp1 * tmp1;
-  double tmp3 = tmp1 * tmp2;
+  const T u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
+  T tmp1 = 1.0e+00 / uu[0];
+  T tmp2 = tmp1 * tmp1;
+  T tmp3 = tmp1 * tmp2;
   for (int m = 0; m < 5; ++m)
     for (int n = 0; n < 5; ++n) D[m][n] = 0.0;
   D[0][0] = 1.0e+00 + dt * 2.0e+00 * (tx1 * dx1 + ty1 * dy1 + tz1 * dz1);
@@ -691,12 +701,13 @@ static void block_d(const Jac& J, const double* uu, double D[5][5]) {
 
 // xi-neighbour block from u at (i-1) for jacld's c (src/LU/jacld.jl:288-364)
 // or at (i+1) for jacu's a (src/LU/jacu.jl, the udx block).
-static void block_x(const Jac& J, double s, const double* uu, double B[5][5]) {
+template <class T>
+static void block_x(const Jac& J, double s, const T* uu, T B[5][5]) {
   const double dt = J.dt, sdt = s * J.dt, tx1 = J.tx1, tx2 = J.tx2, r43 = J.r43, c1345 = J.c1345, c34 = J.c34;
-  const double u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
-  double tmp1 = 1.0e+00 / uu[0];
-  double tmp2 = tmp1 * tmp1;
-  double tmp3 = tmp1 * tmp2;
+  const T u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
+  T tmp1 = 1.0e+00 / uu[0];
+  T tmp2 = tmp1 * tmp1;
+  T tmp3 = tmp1 * tmp2;
   for (int m = 0; m < 5; ++m)
     for (int n = 0; n < 5; ++n) B[m][n] = 0.0;
   B[0][0] = -dt * tx1 * dx1;
@@ -726,12 +737,13 @@ static void block_x(const Jac& J, double s, const double* uu, double B[5][5]) {
 
 // eta-neighbour block from u at (j-1) for jacld's b (src/LU/jacld.jl:207-283)
 // or at (j+1) for jacu's b.
-static void block_y(const Jac& J, double s, const double* uu, double B[5][5]) {
+template <class T>
+static void block_y(const Jac& J, double s, const T* uu, T B[5][5]) {
   const double dt = J.dt, sdt = s * J.dt, ty1 = J.ty1, ty2 = J.ty2, r43 = J.r43, c1345 = J.c1345, c34 = J.c34;
-  const double u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
-  double tmp1 = 1.0e+00 / uu[0];
-  double tmp2 = tmp1 * tmp1;
-  double tmp3 = tmp1 * tmp2;
+  const T u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
+  T tmp1 = 1.0e+00 / uu[0];
+  T tmp2 = tmp1 * tmp1;
+  T tmp3 = tmp1 * tmp2;
   for (int m = 0; m < 5; ++m)
     for (int n = 0; n < 5; ++n) B[m][n] = 0.0;
   B[0][0] = -dt * ty1 * dy1;
@@ -761,12 +773,13 @@ static void block_y(const Jac& J, double s, const double* uu, double B[5][5]) {
 
 // zeta-neighbour block from u at (k-1) for jacld's a (src/LU/jacld.jl:128-202)
 // or at (k+1) for jacu's c.
-static void block_z(const Jac& J, double s, const double* uu, double B[5][5]) {
+template <class T>
+static void block_z(const Jac& J, double s, const T* uu, T B[5][5]) {
   const double dt = J.dt, sdt = s * J.dt, tz1 = J.tz1, tz2 = J.tz2, r43 = J.r43, c1345 = J.c1345, c34 = J.c34;
-  const double u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
-  double tmp1 = 1.0e+00 / uu[0];
-  double tmp2 = tmp1 * tmp1;
-  double tmp3 = tmp1 * tmp2;
+  const T u2 = uu[1], u3 = uu[2], u4 = uu[3], u5 = uu[4];
+  T tmp1 = 1.0e+00 / uu[0];
+  T tmp2 = tmp1 * tmp1;
+  T tmp3 = tmp1 * tmp2;
   for (int m = 0; m < 5; ++m)
     for (int n = 0; n < 5; ++n) B[m][n] = 0.0;
   B[0][0] = -dt * tz1 * dz1;
@@ -794,15 +807,17 @@ static void block_z(const Jac& J, double s, const double* uu, double B[5][5]) {
   B[4][4] = sdt * tz2 * (c1 * (u4 * tmp1)) - dt * tz1 * c1345 * tmp1 - dt * tz1 * dz5;
 }
 
-static void store(Blocks& X, int i, const double B[5][5]) {
+template <class T>
+static void store(BlocksT<T>& X, int i, const T B[5][5]) {
   for (int m = 1; m <= 5; ++m)
     for (int n = 1; n <= 5; ++n) X(m, n, i) = B[m - 1][n - 1];
 }
 
 // jacld(j, k): d at (i,j,k), a from (i,j,k-1), b from (i,j-1,k), c from
 // (i-1,j,k) for i = ist..iend.
-static void jacld(Zone& z, const Jac& J, int j, int k) {
-  double B[5][5];
+template <class T>
+static void jacld(ZoneT<T>& z, const Jac& J, int j, int k) {
+  T B[5][5];
   for (int i = 2; i <= z.nx - 1; ++i) {
     block_d(J, z.u.at(i, j, k), B); store(z.d, i, B);
     block_z(J, -1.0, z.u.at(i, j, k - 1), B); store(z.a, i, B);
@@ -813,8 +828,9 @@ static void jacld(Zone& z, const Jac& J, int j, int k) {
 
 // jacu(j, k): d at (i,j,k), a from (i+1,j,k), b from (i,j+1,k), c from
 // (i,j,k+1).
-static void jacu(Zone& z, const Jac& J, int j, int k) {
-  double B[5][5];
+template <class T>
+static void jacu(ZoneT<T>& z, const Jac& J, int j, int k) {
+  T B[5][5];
   for (int i = 2; i <= z.nx - 1; ++i) {
     block_d(J, z.u.at(i, j, k), B); store(z.d, i, B);
     block_x(J, 1.0, z.u.at(i + 1, j, k), B); store(z.a, i, B);
@@ -825,15 +841,16 @@ static void jacu(Zone& z, const Jac& J, int j, int k) {
 
 // blts, src/LU/blts.jl:9-152: the lower-triangular sweep of one (j,k) row
 // with ldz = a, ldy = b, ldx = c.
-static void blts(Zone& z, int j, int k, double omega) {
-  Field& v = z.rsd;
+template <class T>
+static void blts(ZoneT<T>& z, int j, int k, double omega) {
+  FieldT<T>& v = z.rsd;
   const int ist = 2, iend = z.nx - 1;
   for (int i = ist; i <= iend; ++i)
     for (int m = 1; m <= 5; ++m)
       v(m, i, j, k) -= omega * (z.a(m, 1, i) * v(1, i, j, k - 1) + z.a(m, 2, i) * v(2, i, j, k - 1) +
                                 z.a(m, 3, i) * v(3, i, j, k - 1) + z.a(m, 4, i) * v(4, i, j, k - 1) +
                                 z.a(m, 5, i) * v(5, i, j, k - 1));
-  double t[6][6];
+  T t[6][6];
   for (int i = ist; i <= iend; ++i) {
     for (int m = 1; m <= 5; ++m)
       v(m, i, j, k) -= omega * (z.b(m, 1, i) * v(1, i, j - 1, k) + z.c(m, 1, i) * v(1, i - 1, j, k) +
@@ -843,12 +860,12 @@ static void blts(Zone& z, int j, int k, double omega) {
                                 z.b(m, 5, i) * v(5, i, j - 1, k) + z.c(m, 5, i) * v(5, i - 1, j, k));
     for (int m = 1; m <= 5; ++m)
       for (int n = 1; n <= 5; ++n) t[m][n] = z.d(m, n, i);
-    double* w = z.rsd.at(i, j, k) - 1;  // w[1..5]
+    T* w = z.rsd.at(i, j, k) - 1;  // w[1..5]
     // Forward elimination without pivoting, then back substitution.
     for (int p = 1; p <= 4; ++p) {
-      double tmp1 = 1.0e+00 / t[p][p];
+      T tmp1 = 1.0e+00 / t[p][p];
       for (int r = p + 1; r <= 5; ++r) {
-        double tmp = tmp1 * t[r][p];
+        T tmp = tmp1 * t[r][p];
         for (int c = p + 1; c <= 5; ++c) t[r][c] -= tmp * t[p][c];
         w[r] -= w[p] * tmp;
       }
@@ -868,15 +885,16 @@ static void blts(Zone& z, int j, int k, double omega) {
 // buts, src/LU/buts.jl:9-148: the upper-triangular sweep of one (j,k) row
 // with udx = a, udy = b, udz = c, processed for i = iend down to ist.  With
 // npb false the signs of :97, :126, :129 and :132 are kept as written.
-static void buts(Zone& z, int j, int k, double omega, bool npb) {
-  Field& v = z.rsd;
+template <class T>
+static void buts(ZoneT<T>& z, int j, int k, double omega, bool npb) {
+  FieldT<T>& v = z.rsd;
   const int ist = 2, iend = z.nx - 1;
   for (int i = iend; i >= ist; --i)
     for (int m = 1; m <= 5; ++m)
       z.TV(m, i) = omega * (z.c(m, 1, i) * v(1, i, j, k + 1) + z.c(m, 2, i) * v(2, i, j, k + 1) +
                             z.c(m, 3, i) * v(3, i, j, k + 1) + z.c(m, 4, i) * v(4, i, j, k + 1) +
                             z.c(m, 5, i) * v(5, i, j, k + 1));
-  double t[6][6];
+  T t[6][6];
   for (int i = iend; i >= ist; --i) {
     for (int m = 1; m <= 5; ++m)
       z.TV(m, i) += omega * (z.b(m, 1, i) * v(1, i, j + 1, k) + z.a(m, 1, i) * v(1, i + 1, j, k) +
@@ -886,11 +904,11 @@ static void buts(Zone& z, int j, int k, double omega, bool npb) {
                              z.b(m, 5, i) * v(5, i, j + 1, k) + z.a(m, 5, i) * v(5, i + 1, j, k));
     for (int m = 1; m <= 5; ++m)
       for (int n = 1; n <= 5; ++n) t[m][n] = z.d(m, n, i);
-    double* w = &z.TV(1, i) - 1;  // w[1..5]
+    T* w = &z.TV(1, i) - 1;  // w[1..5]
     for (int p = 1; p <= 4; ++p) {
-      double tmp1 = 1.0e+00 / t[p][p];
+      T tmp1 = 1.0e+00 / t[p][p];
       for (int r = p + 1; r <= 5; ++r) {
-        double tmp = tmp1 * t[r][p];
+        T tmp = tmp1 * t[r][p];
         for (int c = p + 1; c <= 5; ++c) {
           if (!npb && p == 2 && r == 5 && c == 5)
             t[r][c] -= -tmp * t[p][c];  // :97 as written
@@ -923,14 +941,16 @@ static void buts(Zone& z, int j, int k, double omega, bool npb) {
 }
 
 // The five parts of ssor (src/LU/ssor.jl:9-254).
-static void ssor_scale(Zone& z, double dt) {
+template <class T>
+static void ssor_scale(ZoneT<T>& z, double dt) {
   for (int k = 2; k <= z.nz - 1; ++k)
     for (int j = 2; j <= z.ny - 1; ++j)
       for (int i = 2; i <= z.nx - 1; ++i)
         for (int m = 1; m <= 5; ++m) z.rsd(m, i, j, k) = dt * z.rsd(m, i, j, k);
 }
 
-static void ssor_lower(Zone& z, const Jac& J, double omega) {
+template <class T>
+static void ssor_lower(ZoneT<T>& z, const Jac& J, double omega) {
   for (int k = 2; k <= z.nz - 1; ++k)
     for (int j = 2; j <= z.ny - 1; ++j) {
       jacld(z, J, j, k);
@@ -938,7 +958,8 @@ static void ssor_lower(Zone& z, const Jac& J, double omega) {
     }
 }
 
-static void ssor_upper(Zone& z, const Jac& J, double omega, bool npb) {
+template <class T>
+static void ssor_upper(ZoneT<T>& z, const Jac& J, double omega, bool npb) {
   for (int k = z.nz - 1; k >= 2; --k)
     for (int j = z.ny - 1; j >= 2; --j) {
       jacu(z, J, j, k);
@@ -946,7 +967,8 @@ static void ssor_upper(Zone& z, const Jac& J, double omega, bool npb) {
     }
 }
 
-static void ssor_update(Zone& z, double omega) {
+template <class T>
+static void ssor_update(ZoneT<T>& z, double omega) {
   double tmp = 1.0e+00 / (omega * (2.0e+00 - omega));
   for (int k = 2; k <= z.nz - 1; ++k)
     for (int j = 2; j <= z.ny - 1; ++j)
@@ -978,20 +1000,21 @@ static std::vector<LuNbr> lu_neighbors(int x_zones, int y_zones) {
 // planes first (:69-181), then each zone unpacks south, north, west, east in
 // that order (:183-288), so the j faces win at the corners.  Planes are
 // k = 1..nz-2 as written, or k = 2..nz-1 with LU_NPB_EXCH.
-static void exch_qbc(std::vector<Zone>& Z, const std::vector<LuNbr>& N, bool npb) {
+template <class T>
+static void exch_qbc(std::vector<ZoneT<T>>& Z, const std::vector<LuNbr>& N, bool npb) {
   const int nzn = (int)Z.size();
   // Send buffers, indexed by the sending zone: [0] to east (j = ny-1),
   // [1] to west (j = 2), [2] to south (i = nx-1), [3] to north (i = 2).
-  std::vector<std::vector<double>> out((size_t)nzn * 4);
+  std::vector<std::vector<T>> out((size_t)nzn * 4);
   for (int z = 0; z < nzn; ++z) {
-    Zone& a = Z[z];
+    ZoneT<T>& a = Z[z];
     int k0 = npb ? 2 : 1, k1 = npb ? a.nz - 1 : a.nz - 2;
-    auto pack_j = [&](std::vector<double>& b, int j) {
+    auto pack_j = [&](std::vector<T>& b, int j) {
       for (int k = k0; k <= k1; ++k)
         for (int i = 1; i <= a.nx; ++i)
           for (int m = 1; m <= 5; ++m) b.push_back(a.u(m, i, j, k));
     };
-    auto pack_i = [&](std::vector<double>& b, int i) {
+    auto pack_i = [&](std::vector<T>& b, int i) {
       for (int k = k0; k <= k1; ++k)
         for (int j = 1; j <= a.ny; ++j)
           for (int m = 1; m <= 5; ++m) b.push_back(a.u(m, i, j, k));
@@ -1002,15 +1025,15 @@ static void exch_qbc(std::vector<Zone>& Z, const std::vector<LuNbr>& N, bool npb
     pack_i(out[(size_t)z * 4 + 3], 2);
   }
   for (int z = 0; z < nzn; ++z) {
-    Zone& a = Z[z];
+    ZoneT<T>& a = Z[z];
     int k0 = npb ? 2 : 1, k1 = npb ? a.nz - 1 : a.nz - 2;
-    auto unpack_i = [&](const std::vector<double>& b, int i) {
+    auto unpack_i = [&](const std::vector<T>& b, int i) {
       size_t p = 0;
       for (int k = k0; k <= k1; ++k)
         for (int j = 1; j <= a.ny; ++j)
           for (int m = 1; m <= 5; ++m) a.u(m, i, j, k) = b[p++];
     };
-    auto unpack_j = [&](const std::vector<double>& b, int j) {
+    auto unpack_j = [&](const std::vector<T>& b, int j) {
       size_t p = 0;
       for (int k = k0; k <= k1; ++k)
         for (int i = 1; i <= a.nx; ++i)
@@ -1288,6 +1311,63 @@ void oracle_lu_set_exact(void* h, int z) {
   for (int k = 1; k <= a.nz; ++k)
     for (int j = 1; j <= a.ny; ++j)
       for (int i = 1; i <= a.nx; ++i) exact(i, j, k, a.u.at(i, j, k), a.nx, a.ny, a.nz);
+}
+
+// Forward-mode dual numbers through nsteps LU iterations from the current u
+// and frct: out[0] = loss = sum(w*u_N^2) over every point, out[1+d] its
+// derivative along (vu[d] on u, vf[d] on frct), both over every point.  rsd
+// is first recomputed as rhs(u) (what it holds at an iteration's start after
+// setup or an earlier iteration), so the loss is a function of u and frct
+// alone; each iteration is oracle_lu_step's.  Arrays in the whole-run layout
+// (every zone's [k][j][i][m], zone after zone); the state is left unchanged.
+void oracle_lu_dual(void* h, int nsteps, int ndir, const double* w, const double* vu,
+                    const double* vf, double* out) {
+  constexpr int ND = 4;
+  using D = Dual<ND>;
+  Run* r = (Run*)h;
+  const int nzn = (int)r->Z.size();
+  std::vector<size_t> zoff(nzn + 1, 0);
+  for (int z = 0; z < nzn; ++z) zoff[z + 1] = zoff[z] + r->Z[z].u.v.size();
+  for (int d0 = 0; d0 < std::max(ndir, 1); d0 += ND) {
+    std::vector<ZoneT<D>> Z(nzn);
+    for (int z = 0; z < nzn; ++z) {
+      Zone& a = r->Z[z];
+      ZoneT<D>& b = Z[z];
+      b.nx = a.nx; b.ny = a.ny; b.nz = a.nz;
+      b.alloc();
+      for (size_t q = 0; q < a.u.v.size(); ++q) {
+        b.u.v[q] = D(a.u.v[q]);
+        b.frct.v[q] = D(a.frct.v[q]);
+        for (int e = 0; e < ND && d0 + e < ndir; ++e) {
+          const size_t off = (size_t)(d0 + e) * zoff[nzn] + zoff[z] + q;
+          b.u.v[q].d[e] = vu[off];
+          b.frct.v[q].d[e] = vf[off];
+        }
+      }
+      rhs(b, r->C[z]);
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      exch_qbc(Z, r->N, r->flags & LU_NPB_EXCH);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(r->nthreads)
+      for (int z = 0; z < nzn; ++z) {
+        ZoneT<D>& a = Z[z];
+        Jac J = make_jac(r->C[z], r->dt);
+        ssor_scale(a, r->dt);
+        ssor_lower(a, J, r->omega);
+        ssor_upper(a, J, r->omega, r->flags & LU_NPB_BUTS);
+        ssor_update(a, r->omega);
+        rhs(a, r->C[z]);
+      }
+    }
+    D L(0.0);
+    for (int z = 0; z < nzn; ++z)
+      for (size_t q = 0; q < Z[z].u.v.size(); ++q) {
+        const D& x = Z[z].u.v[q];
+        L = L + w[zoff[z] + q] * (x * x);
+      }
+    out[0] = L.v;
+    for (int e = 0; e < ND && d0 + e < ndir; ++e) out[1 + d0 + e] = L.d[e];
+  }
 }
 
 }  // extern "C"

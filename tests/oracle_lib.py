@@ -61,7 +61,7 @@ def load():
         getattr(L, f"oracle_lu_{n}").argtypes = a
         getattr(L, f"oracle_lu_{n}").restype = None
     L.oracle_lu_nzones.argtypes = [vp]
-    for b in ("bt", "sp"):
+    for b in ("bt", "sp", "lu"):
         getattr(L, f"oracle_{b}_dual").argtypes = [vp, i, i, dp, dp, dp, dp]
         getattr(L, f"oracle_{b}_dual").restype = None
     _lib = L
@@ -241,6 +241,20 @@ class OracleLU:
 
     def set_exact(self, z):
         self.L.oracle_lu_set_exact(self.h, z)
+
+    def dual_gradient(self, nsteps, w, vu, vf):
+        """Forward-mode dual numbers through nsteps iterations from the
+        current u and frct (rsd recomputed as rhs(u) first): returns (loss, d)
+        with loss = sum(w*u_N**2) and d[i] its derivative along (vu[i] on u,
+        vf[i] on frct), every point.  Arrays in the whole-run layout; the state
+        is left unchanged."""
+        vu = np.ascontiguousarray(np.atleast_2d(vu), dtype=np.float64)
+        vf = np.ascontiguousarray(np.atleast_2d(vf), dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        nd = vu.shape[0]
+        out = np.zeros(1 + nd)
+        self.L.oracle_lu_dual(self.h, int(nsteps), nd, _dp(w), _dp(vu), _dp(vf), _dp(out))
+        return out[0], out[1:]
 
 
 def oracle_for(bench):

@@ -132,10 +132,3 @@ def test_adjoint_matches_dual_numbers_class_A(gpu):
     assert np.abs(got / d - 1).max() < 1e-9, np.abs(got / d - 1)
     ctx.close()
 
-
-def test_adjoint_is_not_built_for_lu(gpu):
-    zones, p = gpu.zones_for("LU", "S")
-    ctx = gpu.Context("LU", zones, p.dt, omega=1.2)
-    with pytest.raises(gpu.NpbmzError, match="BT and SP"):
-        ctx.adjoint(2, 1, np.zeros(ctx.size))
-    ctx.close()

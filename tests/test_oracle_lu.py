@@ -252,3 +252,42 @@ def test_norms_recomputed_with_numpy():
         assert abs(per[z, 10] - (f1 + f2 + f3)) <= 1e-12 * abs(f1 + f2 + f3)
     assert np.allclose(xcr, per[:, 5:10].sum(axis=0), rtol=1e-14)
     assert np.allclose(xce, per[:, :5].sum(axis=0), rtol=1e-14)
+
+
+def test_dual_numbers_match_central_differences():
+    """The LU oracle's forward-mode dual run (the LU adjoint's reference
+    gradient) against central differences of its own double-precision run:
+    class S, 5 iterations with NPB's exchange planes, two directions on u and
+    frct over every point (frct's boundary values enter through rsd = -frct
+    there), the loss difference summed point by point at h = 1e-4."""
+    zones, p = P.zones_for("LU", "S")
+    o = OracleLU.for_class(p, flags=NPB)
+    r = np.random.Generator(np.random.Philox(13))
+    shapes = [(zones.nz[z], zones.ny[z], zones.nx[z], 5) for z in range(zones.count)]
+    n = sum(int(np.prod(s)) for s in shapes)
+    w = r.random(n)
+    vu = r.standard_normal((2, n)) * 1e-2
+    vf = r.standard_normal((2, n))
+    L, d = o.dual_gradient(5, w, vu, vf)
+    u0 = [o.get(z, 0) for z in range(zones.count)]
+    f0 = [o.get(z, 1) for z in range(zones.count)]
+
+    def final(du, df):
+        off = 0
+        for z, s in enumerate(shapes):
+            k = int(np.prod(s))
+            o.set(z, 0, u0[z] + du[off:off + k].reshape(s))
+            o.set(z, 1, f0[z] + df[off:off + k].reshape(s))
+            off += k
+        o.phase(5)  # rsd = rhs(u)
+        o.step(5)
+        return np.concatenate([o.get(z, 0).ravel() for z in range(zones.count)])
+
+    base = final(0 * vu[0], 0 * vf[0])
+    assert abs(L / np.sum(w * base * base) - 1) < 1e-13
+    h = 1e-4
+    for q in range(2):
+        up, um = final(h * vu[q], h * vf[q]), final(-h * vu[q], -h * vf[q])
+        fd = np.sum(w * (up - um) * (up + um)) / (2 * h)
+        assert abs(fd / d[q] - 1) < 1e-7, (fd, d[q])
+    o.close()
