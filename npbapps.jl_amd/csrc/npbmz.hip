@@ -735,6 +735,11 @@ int k_sp_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const doub
   hipLaunchKernelGGL(fast::sp::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us,
                      x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
   CK(hipGetLastError());
+  const Table& ti = c.t_int;
+  if (ti.n > 0)
+    hipLaunchKernelGGL((fast::sp::adj_deriv_kernel<DIR, kPtNT>), dim3(ti.n), dim3(kPtNT), 0,
+                       c.stream, us, x, lam, ubar, c.zd, ti.d, c.K);
+  CK(hipGetLastError());
   return NPBMZ_OK;
 }
 
