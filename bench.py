@@ -130,6 +130,33 @@ def class_run(P, bench, cls, npb_exchange=False):
             "schedule": r["schedule"]}
 
 
+SIDE_WORKLOADS = {
+    "sp-a": ("SP", "A"), "sp-b": ("SP", "B"), "lu-a": ("LU", "A"), "lu-b": ("LU", "B"),
+    "bt-a": ("BT", "A"),
+}
+
+
+def side_workload(P, name, tune_ms):
+    """One secondary workload on its own (for its own timing and profile): a
+    whole class run in graph mode with the library's default schedule (LU
+    with NPB's exchange planes), or one of the three adjoint cases."""
+    if name == "bt-a-adjoint":
+        return {"workload": name, **adjoint_case(P)}
+    if name == "sp-a-adjoint":
+        return {"workload": name, **sp_adjoint_case(P)}
+    if name == "lu-a-adjoint":
+        return {"workload": name, **lu_adjoint_case(P)}
+    bench, cls = SIDE_WORKLOADS[name]
+    zones, p = P.zones_for(bench, cls)
+    r = P.go(bench, cls, mode="graph", verbose=False, npb_exchange=bench == "LU",
+             tune_ms=tune_ms)
+    t = r["time_s"] / p.niter
+    return {"workload": f"{bench}-MZ class {cls}, {zones.count} zones, {p.niter} steps",
+            "mops": round(r["mops"], 2), "ms_per_step": round(t * 1e3, 5),
+            "grid_pt_steps_per_s": round(zones.npoints / t, 1), "verified": r["verified"],
+            "schedule": r["schedule"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +173,11 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain copy kernel run for the achievable-bandwidth "
                          "reference and counter calibration (0 = skip)")
+    ap.add_argument("--workload", default="bt-b",
+                    choices=["bt-b", *SIDE_WORKLOADS, "bt-a-adjoint", "sp-a-adjoint",
+                             "lu-a-adjoint"],
+                    help="bt-b (default): the headline line below; any other runs that "
+                         "workload alone and prints its own JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -154,6 +186,9 @@ def main():
         sys.exit(2)
 
     import npbapps_amd as P
+    if args.workload != "bt-b":
+        print(json.dumps(side_workload(P, args.workload, args.tune_ms)))
+        return
     zones, p = P.zones_for("BT", "B")
     K, W = args.steps, args.warmup
     mflop_step = P.mflops_per_step("BT", zones)

@@ -164,8 +164,11 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 
 /* Set one schedule knob directly (the tuner's candidates, for tests and
  * profiling).  No counterpart in the reference. */
-#define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone */
-#define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup */
+#define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone
+                                    (default: 1 for zones of at most 32768 interior
+                                    points, else 0) */
+#define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup
+                                    (default 512) */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
 #define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
                                     with a builder and an eliminator wave (default 2) */

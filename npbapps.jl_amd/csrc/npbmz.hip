@@ -982,7 +982,17 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   c.west.assign(p->iz_west, p->iz_west + n); c.east.assign(p->iz_east, p->iz_east + n);
   c.south.assign(p->iz_south, p->iz_south + n); c.north.assign(p->iz_north, p->iz_north + n);
   c.K = make_constants(c.bench, c.dt, c.gx, c.gy, c.gz, c.x_zones, c.y_zones);
-  if (c.bench == NPBMZ_LU) c.lk = make_lu_constants(c.dt, c.omega, c.nx[0], c.ny[0], c.nz[0]);
+  if (c.bench == NPBMZ_LU) {
+    c.lk = make_lu_constants(c.dt, c.omega, c.nx[0], c.ny[0], c.nz[0]);
+    // default sweep schedule by zone size, as measured on MI355X: one
+    // 512-thread workgroup per zone while a zone's interior is small (class
+    // A, 30x30x14: 0.54 ms per iteration against 0.58 with 256 threads),
+    // hyperplane launches once it is large (class B, 74x50x15: 1.2 ms per
+    // iteration against 2.1 with a workgroup per zone)
+    const int64_t interior = int64_t(c.nx[0] - 2) * (c.ny[0] - 2) * (c.nz[0] - 2);
+    if (interior > 32768) c.lu_sweep = kSweepPlane;
+    else c.lu_nt = 512;
+  }
   c.zh.resize(n);
   int64_t base = 0;
   for (int z = 0; z < n; ++z) {
