@@ -26,3 +26,22 @@ def build_library(verbose=False):
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
     return OUT
+
+
+HARNESS_SRC = os.path.join(HERE, "harness", "npbmz_run.c")
+HARNESS_OUT = os.path.join(HERE, "harness", "npbmz_run")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_harness(verbose=False):
+    """The C caller of the ABI (harness/npbmz_run.c), linked against the
+    in-tree libnpbmz with an rpath to it."""
+    cc = os.environ.get("CC", "gcc")
+    cmd = [cc, "-O2", "-std=c11", "-Wall", "-Wextra", "-I", INCLUDE, HARNESS_SRC, "-L", HERE,
+           "-lnpbmz", "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", HARNESS_OUT]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{cc} failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
+    return HARNESS_OUT

@@ -78,3 +78,35 @@ def test_missing_library_fails_loudly(tmp_path):
             native_mod.load(str(tmp_path / "absent.so"))
         finally:
             native_mod._lib = old
+
+
+def test_header_is_plain_c_and_the_c_harness_compiles(tmp_path):
+    """include/npbmz.h is usable from C11 (the language a cgo, JNI or Julia
+    ccall binding sees): the C caller compiles against it with warnings as
+    errors."""
+    import subprocess
+    src = os.path.join(ROOT, "npbapps.jl_amd", "harness", "npbmz_run.c")
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                        "-Wno-vla", "-I", os.path.join(ROOT, "include"), "-c", src, "-o",
+                        str(tmp_path / "h.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("bench", ["bt", "sp", "lu"])
+def test_c_harness_zone_split_matches_the_python_one(bench):
+    """The C caller's zone split (src/BT/zone_setup.jl, LU rounding from
+    src/LU/zone_setup.jl) against npbapps_amd.zones for classes S, W, A, B;
+    its zones mode prints the table without touching a device."""
+    import json
+    import subprocess
+    from npbapps_amd import build
+    if not os.path.exists(native.LIB_PATH):
+        pytest.skip("libnpbmz is not built")
+    exe = build.build_harness()
+    for cls in "SWAB":
+        out = subprocess.run([exe, bench, cls, "zones"], capture_output=True, text=True,
+                             check=True).stdout
+        got = json.loads(out)
+        z, _ = P.zones_for(bench.upper(), cls)
+        for key in ("nx", "ny", "nz", "west", "east", "south", "north"):
+            assert got[key] == list(getattr(z, key)), (bench, cls, key)
