@@ -791,7 +791,38 @@ int lu_step_store(npbmz_ctx* ctx, double* un, double* ue, double* r1, double* r2
   return k_lu_rhs(ctx, -1);
 }
 
-constexpr int kAdjSweepNT = 256;  // threads of a transposed-sweep workgroup (one per zone)
+// Both transposed sweeps in the context's LU sweep schedule: one workgroup
+// per zone (of lu_nt threads) walking the hyperplanes, or one launch per
+// hyperplane; ascending for the transposed upper sweep, descending for the
+// transposed lower one.
+template <bool ASC>
+int k_lu_adj_sweep(npbmz_ctx* ctx, const double* ue, const double* ubar, double* nu, double* rho) {
+  auto& c = *ctx;
+  if (c.lu_sweep == kSweepZone) {
+    auto go = [&](auto kern, int nt) {
+      hipLaunchKernelGGL(kern, dim3(c.nz_), dim3(nt), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
+    };
+    switch (c.lu_nt) {
+      case 128: go(fast::lu::adj_sweep_zone_kernel<ASC, 128>, 128); break;
+      case 512: go(fast::lu::adj_sweep_zone_kernel<ASC, 512>, 512); break;
+      default: go(fast::lu::adj_sweep_zone_kernel<ASC, 256>, 256); break;
+    }
+    CK(hipGetLastError());
+    return NPBMZ_OK;
+  }
+  const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
+  const int ns = NI + NJ + NK - 2;
+  for (int q = 0; q < ns; ++q) {
+    const int s = ASC ? q : ns - 1 - q;
+    const int c_lo = std::max(0, s - (NI - 1) - (NJ - 1)), c_hi = std::min(NK - 1, s);
+    const int npair = (c_hi - c_lo + 1) * NJ;
+    const dim3 grid((npair + kPlaneNT - 1) / kPlaneNT, c.nz_);
+    hipLaunchKernelGGL((fast::lu::adj_sweep_plane_kernel<ASC, kPlaneNT>), grid, dim3(kPlaneNT), 0,
+                       c.stream, ue, ubar, nu, rho, c.zd, s, c.lk);
+    CK(hipGetLastError());
+  }
+  return NPBMZ_OK;
+}
 
 // Reverse one LU iteration from its stored forward data (un, ue: u before
 // and after the exchange; r1, r2: the sweeps' outputs): ubar holds dL/du
@@ -803,12 +834,9 @@ int lu_adj_step(npbmz_ctx* ctx, const double* un, const double* ue, const double
   const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
   CK(hipMemsetAsync(nu, 0, fb, c.stream));
   CK(hipMemsetAsync(rho, 0, fb, c.stream));
-  hipLaunchKernelGGL((fast::lu::adj_sweep_zone_kernel<true, kAdjSweepNT>), dim3(c.nz_),
-                     dim3(kAdjSweepNT), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
-  CK(hipGetLastError());
-  hipLaunchKernelGGL((fast::lu::adj_sweep_zone_kernel<false, kAdjSweepNT>), dim3(c.nz_),
-                     dim3(kAdjSweepNT), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
-  CK(hipGetLastError());
+  int rc;
+  if ((rc = k_lu_adj_sweep<true>(ctx, ue, ubar, nu, rho))) return rc;
+  if ((rc = k_lu_adj_sweep<false>(ctx, ue, ubar, nu, rho))) return rc;
   const Table& t = c.t_all;
   hipLaunchKernelGGL(fast::lu::adj_blocks_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, ue,
                      r1, r2, nu, rho, ubar, fbar, c.zd, t.d, c.lk);
