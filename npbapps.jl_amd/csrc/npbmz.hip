@@ -644,8 +644,12 @@ int k_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double*
   if (t.n <= 0) return NPBMZ_OK;
   int grid = t.n;
   if (c.solve_grid[DIR] > 0) grid = std::min(t.n, c.solve_grid[DIR]);
-  hipLaunchKernelGGL(fast::bt::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us,
-                     gin, lam, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  if ((c.solve_twist >> DIR) & 1)
+    hipLaunchKernelGGL(fast::bt::adj_solve_twist_kernel<DIR>, dim3(grid), dim3(2 * kLineNT), 0,
+                       c.stream, us, gin, lam, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  else
+    hipLaunchKernelGGL(fast::bt::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream,
+                       us, gin, lam, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
   CK(hipGetLastError());
   const Table& ti = c.t_int;
   if (ti.n > 0)

@@ -31,13 +31,17 @@ def _interior_mask(zones):
     return np.concatenate(parts)
 
 
-@pytest.mark.parametrize("nsteps,ckpt", [(10, 3), (6, 20)])
-def test_adjoint_matches_dual_numbers_class_S(gpu, nsteps, ckpt):
+@pytest.mark.parametrize("nsteps,ckpt,twist", [(10, 3, None), (6, 20, None), (6, 4, 0)])
+def test_adjoint_matches_dual_numbers_class_S(gpu, nsteps, ckpt, twist):
     """Reverse mode on the GPU against forward mode on the CPU oracle: the
     directional derivatives along 8 random (u0, forcing) directions agree to
-    1e-9 (both are exact up to rounding; the GPU runs the FMA kernels)."""
+    1e-9 (both are exact up to rounding; the GPU runs the FMA kernels).
+    twist = 0 runs the forward and the transposed solves one wave per 64
+    lines instead of the default two-sided ones."""
     zones, p = gpu.zones_for("BT", "S")
     ctx = gpu.Context("BT", zones, p.dt)
+    if twist is not None:
+        ctx.set_schedule(gpu.native.SCHED_SOLVE_TWIST, twist)
     ctx.init_exact()
     n = ctx.size
     r = _rng(2024)
