@@ -172,11 +172,14 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
 #define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
                                     with a builder and an eliminator wave (default 2) */
-#define NPBMZ_SCHED_SOLVE_TWIST 4 /* BT, FMA arithmetic only (ignored with
+#define NPBMZ_SCHED_SOLVE_TWIST 4 /* BT and SP, FMA arithmetic only (ignored with
                                      NPBMZ_FLAG_STRICT_FP): bit d set: direction d
                                      eliminates each line from both ends with two
-                                     waves meeting in the middle (overrides PAIR;
-                                     default 7) */
+                                     waves meeting in the middle (BT: overrides
+                                     PAIR, default 7; SP: for directions whose
+                                     lines all have >= 6 points, default on where
+                                     the direction has fewer line tasks of 64
+                                     lines than the device has CUs) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -161,7 +161,8 @@ struct npbmz_ctx {
   // three directions with FMA arithmetic; with strict arithmetic (no
   // two-sided solve) the wave pair in y.
   int solve_pair = 2;             // BT: bit d set: direction d uses solve_pair_kernel
-  int solve_twist = 7;            // BT, FMA only: bit d set: solve_twist_kernel (wins over pair)
+  int solve_twist = 7;            // BT/SP, FMA only: bit d set: solve_twist_kernel (wins over pair)
+  bool sp_twist_ok[3] = {false, false, false};  // SP: every line of direction d has >= 4 unknowns
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
@@ -456,6 +457,8 @@ int k_solve_t(npbmz_ctx* ctx, int zone) {
   };
   if (c.bench == NPBMZ_SP) {
     if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>, kLineNT);
+    else if (((c.solve_twist >> DIR) & 1) && c.sp_twist_ok[DIR])
+      go(fast::sp::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
     else go(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
   } else if (!strict(ctx) && ((c.solve_twist >> DIR) & 1)) {
     go(fast::bt::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
@@ -932,9 +935,13 @@ void describe(npbmz_ctx* c) {
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else if (c->bench == NPBMZ_SP)
-    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d fp=%s", c->nz_,
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d twist=%d fp=%s", c->nz_,
              c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
-             c->solve_grid[1], c->solve_grid[2], strict(c) ? "strict" : "fma");
+             c->solve_grid[1], c->solve_grid[2],
+             strict(c) ? 0
+                       : (c->solve_twist & ((c->sp_twist_ok[0] ? 1 : 0) | (c->sp_twist_ok[1] ? 2 : 0) |
+                                            (c->sp_twist_ok[2] ? 4 : 0))),
+             strict(c) ? "strict" : "fma");
   else
     snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d fp=%s", c->nz_,
              c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
@@ -1014,6 +1021,15 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   c.west.assign(p->iz_west, p->iz_west + n); c.east.assign(p->iz_east, p->iz_east + n);
   c.south.assign(p->iz_south, p->iz_south + n); c.north.assign(p->iz_north, p->iz_north + n);
   c.K = make_constants(c.bench, c.dt, c.gx, c.gy, c.gz, c.x_zones, c.y_zones);
+  if (c.bench == NPBMZ_SP) {
+    // the two-sided solve needs 4 unknown rows on every line of a direction
+    for (int d = 0; d < 3; ++d) {
+      int shortest = 1 << 30;
+      for (int z = 0; z < n; ++z)
+        shortest = std::min(shortest, d == 0 ? c.nx[z] : d == 1 ? c.ny[z] : c.nz[z]);
+      c.sp_twist_ok[d] = shortest - 2 >= 4;
+    }
+  }
   if (c.bench == NPBMZ_LU) {
     c.lk = make_lu_constants(c.dt, c.omega, c.nx[0], c.ny[0], c.nz[0]);
     // default sweep schedule by zone size, as measured on MI355X: one
@@ -1065,6 +1081,18 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   CKC(hipMemset(c.frc, 0, fb));
   CKC(hipMemset(c.rhs, 0, fb));
   if ((rc = build_tables(ctx))) return bad(rc);
+  if (c.bench == NPBMZ_SP) {
+    // default: two-sided solves in the directions with fewer line tasks
+    // than the chip has CUs, as measured on MI355X (class A, at most 225
+    // tasks per direction: 0.112 -> 0.096 ms/step; class B, 360 to 864
+    // tasks: 0.203 -> 0.209, so one wave per 64 lines there)
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    c.solve_twist = 0;
+    for (int d = 0; d < 3; ++d)
+      if (c.sp_twist_ok[d] && c.t_line[d].n < cus) c.solve_twist |= 1 << d;
+  }
   describe(ctx);
 #undef CKC
   *out = ctx;
@@ -1397,8 +1425,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
-    if (c.bench == NPBMZ_BT && !strict(ctx))
-      knobs.push_back({NPBMZ_SCHED_SOLVE_TWIST, {0, 1, 3, 7}});
+    if (!strict(ctx)) knobs.push_back({NPBMZ_SCHED_SOLVE_TWIST, {0, 1, 3, 7}});
     knobs.push_back({NPBMZ_SCHED_SOLVE_GRID, {0, 1024}});
   }
   auto get = [&](int which) -> int {
@@ -1491,8 +1518,8 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       c.solve_pair = value;
       break;
     case NPBMZ_SCHED_SOLVE_TWIST:
-      if (c.bench != NPBMZ_BT || value < 0 || value > 7)
-        return fail(ctx, NPBMZ_EINVAL, "BT two-sided solve mask is 0..7");
+      if (c.bench == NPBMZ_LU || value < 0 || value > 7)
+        return fail(ctx, NPBMZ_EINVAL, "BT/SP two-sided solve mask is 0..7");
       c.solve_twist = value;
       break;
     case NPBMZ_SCHED_SOLVE_GRID: {

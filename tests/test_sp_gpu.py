@@ -102,3 +102,22 @@ def test_perzone_and_graph_agree_class_A(gpu):
         ctx.close()
     for a, b in zip(*res):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("cls", ["S", "W", "A"])
+def test_two_sided_solves_track_the_one_sided_ones(gpu, cls):
+    """The two-sided pentadiagonal solves (two waves per 64 lines meeting in
+    a 4x4 interface system, FMA only) against the one-wave solves: 5 steps
+    from the same state agree to 1e-12 of the field's size."""
+    N = gpu.native
+    zones, p = gpu.zones_for("SP", cls)
+    out = []
+    for twist in (0, 7):
+        ctx = gpu.Context("SP", zones, p.dt)
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, twist)
+        ctx.init_exact()
+        ctx.step(5, "graph")
+        out.append(ctx.get_all(0))
+        ctx.close()
+    a, b = out
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
