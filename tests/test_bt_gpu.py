@@ -238,4 +238,22 @@ def test_errors_are_reported_not_raised_across_the_abi(gpu):
         ctx.put_field(0, 0, np.zeros((2, 2, 2, 5)), (1, 0, 0, 0), (5, 1, 1, 1))
     with pytest.raises(gpu.NpbmzError, match="unknown mode"):
         ctx.step(1, 7)
+    N = gpu.native
+    with pytest.raises(gpu.NpbmzError, match="LU sweep"):
+        ctx.set_schedule(N.SCHED_LU_SWEEP, 1)
+    with pytest.raises(gpu.NpbmzError, match="two-sided"):
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, 9)
+    with pytest.raises(gpu.NpbmzError, match="unknown phase"):
+        ctx.run_phase(17)
+    with pytest.raises(gpu.NpbmzError, match="nsteps"):
+        ctx.phase_times(0)
+    with pytest.raises(gpu.NpbmzError, match="nsteps and ckpt_every"):
+        ctx.adjoint(0, 20, np.zeros(ctx.size))
+    with pytest.raises(gpu.NpbmzError, match="nsteps and ckpt_every"):
+        ctx.adjoint(4, 0, np.zeros(ctx.size))
+    # the context is still usable: a class S run from here verifies
+    ctx.init_exact()
+    ctx.step(p.niter, "graph")
+    xcr, xce, _ = ctx.norms()
+    assert gpu.verify.verify("BT", "S", p.dt, p.niter, xcr, xce)[0]
     ctx.close()
