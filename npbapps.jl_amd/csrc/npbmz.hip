@@ -411,7 +411,7 @@ bool strict(const npbmz_ctx* c) { return (c->flags & NPBMZ_FLAG_STRICT_FP) != 0;
 
 // compute_rhs.  For SP the timestep's rhs phase is compute_rhs + txinvr;
 // plain = true gives compute_rhs alone (SP's verify, src/SP/verify.jl:436).
-int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false) {
+int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false, double* r0out = nullptr) {
   auto& c = *ctx;
   const Table& t = c.t_int;
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
@@ -419,7 +419,7 @@ int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false) {
   const bool tx = c.bench == NPBMZ_SP && !plain;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, c.u, c.frc, c.rhs, c.zd,
-                       t.d + first, c.K);
+                       t.d + first, c.K, tx ? r0out : nullptr);
   };
   if (strict(ctx)) {
     if (tx) go(exact::bt::rhs_kernel<kPtNT, true>);
@@ -443,8 +443,9 @@ int k_add(npbmz_ctx* ctx, int zone) {
   return NPBMZ_OK;
 }
 
+// xraw (SP only) also receives each line's solution before its transform.
 template <int DIR, bool FUSE>
-int k_solve_t(npbmz_ctx* ctx, int zone) {
+int k_solve_t(npbmz_ctx* ctx, int zone, double* xraw = nullptr) {
   auto& c = *ctx;
   const Table& t = c.t_line[DIR];
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
@@ -455,11 +456,15 @@ int k_solve_t(npbmz_ctx* ctx, int zone) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
                        t.d + first, n, c.maxrows[DIR], c.K);
   };
+  auto go_sp = [&](auto kern, int nt) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
+                       t.d + first, n, c.maxrows[DIR], c.K, xraw);
+  };
   if (c.bench == NPBMZ_SP) {
-    if (strict(ctx)) go(exact::sp::solve_kernel<DIR, FUSE>, kLineNT);
+    if (strict(ctx)) go_sp(exact::sp::solve_kernel<DIR, FUSE>, kLineNT);
     else if (((c.solve_twist >> DIR) & 1) && c.sp_twist_ok[DIR])
-      go(fast::sp::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
-    else go(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
+      go_sp(fast::sp::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
+    else go_sp(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
   } else if (!strict(ctx) && ((c.solve_twist >> DIR) & 1)) {
     go(fast::bt::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
   } else if ((c.solve_pair >> DIR) & 1) {
@@ -475,10 +480,10 @@ int k_solve_t(npbmz_ctx* ctx, int zone) {
 
 // x/y/z_solve (BT block tridiagonal, SP pentadiagonal with its inverse
 // transforms); fuse folds add into the z sweep.
-int k_solve(npbmz_ctx* ctx, int dir, bool fuse, int zone) {
-  if (dir == 0) return k_solve_t<0, false>(ctx, zone);
-  if (dir == 1) return k_solve_t<1, false>(ctx, zone);
-  return fuse ? k_solve_t<2, true>(ctx, zone) : k_solve_t<2, false>(ctx, zone);
+int k_solve(npbmz_ctx* ctx, int dir, bool fuse, int zone, double* xraw = nullptr) {
+  if (dir == 0) return k_solve_t<0, false>(ctx, zone, xraw);
+  if (dir == 1) return k_solve_t<1, false>(ctx, zone, xraw);
+  return fuse ? k_solve_t<2, true>(ctx, zone, xraw) : k_solve_t<2, false>(ctx, zone, xraw);
 }
 
 int exch(npbmz_ctx* ctx) {
@@ -687,48 +692,20 @@ int bt_adj_step(npbmz_ctx* ctx, const double* us, const double* x1, const double
 }
 
 // ------------------------------------------------------------ SP adjoint
-// One batched SP step as separate launches, copying the post-exchange u, r0
-// = dt*compute_rhs, and the three solve outputs before their transforms when
-// the targets are given: compute_rhs, txinvr, then per direction the solve
-// without its transform, the transform, and finally add.
+// One batched SP step with the production kernels, keeping what the reverse
+// pass reads when the targets are given: the post-exchange u, r0 =
+// dt*compute_rhs before txinvr, and each direction's line solutions before
+// their transforms.
 int sp_step_store(npbmz_ctx* ctx, double* us, double* r0, double* x1, double* x2, double* x3) {
   auto& c = *ctx;
   const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
-  const Table& ti = c.t_int;
   int rc;
   if ((rc = exch(ctx))) return rc;
   if (us) CK(hipMemcpyAsync(us, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
-  if ((rc = k_rhs(ctx, -1, true))) return rc;
-  if (r0) CK(hipMemcpyAsync(r0, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
-  if (ti.n > 0)
-    hipLaunchKernelGGL(fast::sp::txinvr_kernel<kPtNT>, dim3(ti.n), dim3(kPtNT), 0, c.stream, c.u,
-                       c.rhs, c.zd, ti.d, c.K);
-  CK(hipGetLastError());
-  double* xs[3] = {x1, x2, x3};
-  for (int d = 0; d < 3; ++d) {
-    const Table& t = c.t_line[d];
-    int grid = t.n;
-    if (c.solve_grid[d] > 0) grid = std::min(t.n, c.solve_grid[d]);
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(kLineNT), 0, c.stream, c.u, c.rhs, c.cscr, c.zd,
-                         t.d, t.n, c.maxrows[d], c.K);
-    };
-    if (d == 0) go(fast::sp::solve_kernel<0, false, false>);
-    if (d == 1) go(fast::sp::solve_kernel<1, false, false>);
-    if (d == 2) go(fast::sp::solve_kernel<2, false, false>);
-    CK(hipGetLastError());
-    if (xs[d]) CK(hipMemcpyAsync(xs[d], c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
-    auto tr = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(ti.n), dim3(kPtNT), 0, c.stream, c.u, c.rhs, c.zd, ti.d, c.K);
-    };
-    if (ti.n > 0) {
-      if (d == 0) tr(fast::sp::transform_kernel<0, kPtNT>);
-      if (d == 1) tr(fast::sp::transform_kernel<1, kPtNT>);
-      if (d == 2) tr(fast::sp::transform_kernel<2, kPtNT>);
-    }
-    CK(hipGetLastError());
-  }
-  return k_add(ctx, -1);
+  if ((rc = k_rhs(ctx, -1, false, r0))) return rc;
+  if ((rc = k_solve(ctx, 0, false, -1, x1))) return rc;
+  if ((rc = k_solve(ctx, 1, false, -1, x2))) return rc;
+  return k_solve(ctx, 2, true, -1, x3);
 }
 
 template <int DIR>
