@@ -708,21 +708,29 @@ int sp_step_store(npbmz_ctx* ctx, double* us, double* r0, double* x1, double* x2
   return k_solve(ctx, 2, true, -1, x3);
 }
 
+// The transposed solve of direction DIR and its matrices' derivative; for
+// DIR 0 (given r0 and lamout) also txinvr's adjoint, into lamout.
 template <int DIR>
 int k_sp_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double* gin,
-                   double* lam, double* ubar) {
+                   double* lam, double* ubar, const double* r0 = nullptr,
+                   double* lamout = nullptr) {
   auto& c = *ctx;
   const Table& t = c.t_line[DIR];
   if (t.n <= 0) return NPBMZ_OK;
   int grid = t.n;
   if (c.solve_grid[DIR] > 0) grid = std::min(t.n, c.solve_grid[DIR]);
-  hipLaunchKernelGGL(fast::sp::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream, us,
-                     x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
+  if (((c.solve_twist >> DIR) & 1) && c.sp_twist_ok[DIR])
+    hipLaunchKernelGGL(fast::sp::adj_solve_twist_kernel<DIR>, dim3(grid), dim3(2 * kLineNT), 0,
+                       c.stream, us, x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR],
+                       c.K);
+  else
+    hipLaunchKernelGGL(fast::sp::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream,
+                       us, x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
   CK(hipGetLastError());
   const Table& ti = c.t_int;
   if (ti.n > 0)
     hipLaunchKernelGGL((fast::sp::adj_deriv_kernel<DIR, kPtNT>), dim3(ti.n), dim3(kPtNT), 0,
-                       c.stream, us, x, lam, ubar, c.zd, ti.d, c.K);
+                       c.stream, us, x, lam, ubar, c.zd, ti.d, c.K, r0, lamout);
   CK(hipGetLastError());
   return NPBMZ_OK;
 }
@@ -736,15 +744,13 @@ int sp_adj_step(npbmz_ctx* ctx, const double* us, const double* r0, const double
   int rc;
   if ((rc = k_sp_adj_solve<2>(ctx, us, x3, ubar, lam, ubar))) return rc;
   if ((rc = k_sp_adj_solve<1>(ctx, us, x2, lam, lam, ubar))) return rc;
-  if ((rc = k_sp_adj_solve<0>(ctx, us, x1, lam, lam, ubar))) return rc;
-  const Table& ti = c.t_int;
-  if (ti.n > 0)
-    hipLaunchKernelGGL(fast::sp::adj_txinvr_kernel<kPtNT>, dim3(ti.n), dim3(kPtNT), 0, c.stream,
-                       us, r0, lam, ubar, c.zd, ti.d, c.K);
-  CK(hipGetLastError());
+  // txinvr's adjoint goes to the step workspace rhs, free during the reverse
+  // pass (off the interior it is never read: adj_rhs_kernel reads the
+  // interior only)
+  if ((rc = k_sp_adj_solve<0>(ctx, us, x1, lam, lam, ubar, r0, c.rhs))) return rc;
   const Table& t = c.t_all;
-  hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us, lam,
-                     ubar, fbar, c.zd, t.d, c.K);
+  hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us,
+                     c.rhs, ubar, fbar, c.zd, t.d, c.K);
   CK(hipGetLastError());
   if (c.nz_ > 1 && c.t_face.n > 0) {
     for (int xf = 1; xf >= 0; --xf) {

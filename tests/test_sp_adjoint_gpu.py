@@ -30,14 +30,21 @@ def _interior_mask(zones):
     return np.concatenate(parts)
 
 
-@pytest.mark.parametrize("cls,nsteps,ckpt", [("S", 10, 3), ("S", 6, 20), ("W", 8, 4)])
-def test_adjoint_matches_dual_numbers(gpu, cls, nsteps, ckpt):
+@pytest.mark.parametrize(
+    "cls,nsteps,ckpt,twist",
+    [("S", 10, 3, None), ("S", 6, 20, None), ("W", 8, 4, None), ("S", 6, 4, 0), ("W", 4, 2, 0)],
+)
+def test_adjoint_matches_dual_numbers(gpu, cls, nsteps, ckpt, twist):
     """Reverse mode on the GPU against forward mode on the CPU oracle: the
     directional derivatives along 8 random (u0, forcing) directions agree to
     1e-9 and the loss to 1e-12; the forcing gradient vanishes off the
-    interior; u is back at u0."""
+    interior; u is back at u0.  The default schedule runs the two-sided
+    forward and transposed solves in every direction at these sizes; twist =
+    0 runs both one wave per 64 lines."""
     zones, p = gpu.zones_for("SP", cls)
     ctx = gpu.Context("SP", zones, p.dt)
+    if twist is not None:
+        ctx.set_schedule(gpu.native.SCHED_SOLVE_TWIST, twist)
     ctx.init_exact()
     n = ctx.size
     r = _rng(2025)
