@@ -179,7 +179,9 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                      PAIR, default 7; SP: for directions whose
                                      lines all have >= 6 points, default on where
                                      the direction has fewer line tasks of 64
-                                     lines than the device has CUs) */
+                                     lines than the device has CUs); the
+                                     adjoint's transposed solves follow the
+                                     same bits */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated
