@@ -126,6 +126,7 @@ constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
+constexpr size_t kLuFwdMaxLds = 65536;  // LDS a zone sweep may take to forward hyperplanes
 
 // LU sweep schedules: one launch per hyperplane over all zones, or one
 // workgroup per zone walking all hyperplanes of the sweep.
@@ -166,6 +167,7 @@ struct npbmz_ctx {
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
+  int lu_fwd = 1;                 // per-zone sweep: forward each hyperplane through LDS
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Face* faces = nullptr;
   double* partial = nullptr;
@@ -543,14 +545,28 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
   const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
   const int ns = NI + NJ + NK - 2;
   if (c.lu_sweep == kSweepZone) {
+    // the hyperplane forwarding through LDS when two hyperplanes fit
+    const size_t lds = sizeof(double) * 2 * 5 * (size_t)NJ * NK;
+    const bool fwd = c.lu_fwd && lds <= kLuFwdMaxLds;
     auto go = [&](auto kern, int nt) {
-      hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), 0, c.stream, c.u, c.rhs, c.zd, z0, c.lk);
+      hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), fwd ? lds : 0, c.stream, c.u, c.rhs, c.zd, z0,
+                         c.lk);
     };
+    auto pick = [&](auto kfwd, auto kglobal, int nt) { fwd ? go(kfwd, nt) : go(kglobal, nt); };
     const bool s = strict(ctx);
     switch (c.lu_nt) {
-      case 128: s ? go(exact::lu::sweep_zone_kernel<UPPER, 128>, 128) : go(fast::lu::sweep_zone_kernel<UPPER, 128>, 128); break;
-      case 512: s ? go(exact::lu::sweep_zone_kernel<UPPER, 512>, 512) : go(fast::lu::sweep_zone_kernel<UPPER, 512>, 512); break;
-      default: s ? go(exact::lu::sweep_zone_kernel<UPPER, 256>, 256) : go(fast::lu::sweep_zone_kernel<UPPER, 256>, 256); break;
+      case 128:
+        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 128, true>, exact::lu::sweep_zone_kernel<UPPER, 128, false>, 128)
+          : pick(fast::lu::sweep_zone_kernel<UPPER, 128, true>, fast::lu::sweep_zone_kernel<UPPER, 128, false>, 128);
+        break;
+      case 512:
+        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 512, true>, exact::lu::sweep_zone_kernel<UPPER, 512, false>, 512)
+          : pick(fast::lu::sweep_zone_kernel<UPPER, 512, true>, fast::lu::sweep_zone_kernel<UPPER, 512, false>, 512);
+        break;
+      default:
+        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 256, true>, exact::lu::sweep_zone_kernel<UPPER, 256, false>, 256)
+          : pick(fast::lu::sweep_zone_kernel<UPPER, 256, true>, fast::lu::sweep_zone_kernel<UPPER, 256, false>, 256);
+        break;
     }
     CK(hipGetLastError());
     return NPBMZ_OK;
@@ -914,8 +930,9 @@ int ensure_graph(npbmz_ctx* ctx) {
 void describe(npbmz_ctx* c) {
   char b[256];
   if (c->bench == NPBMZ_LU)
-    snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d exch=%s fp=%s", c->nz_,
+    snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
+             c->lu_sweep == kSweepZone && c->lu_fwd ? " lds-forward" : "",
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else if (c->bench == NPBMZ_SP)
     snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d twist=%d fp=%s", c->nz_,
@@ -1404,7 +1421,8 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   struct Knob { int which; std::vector<int> values; };
   std::vector<Knob> knobs;
   if (c.bench == NPBMZ_LU) {
-    knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}}};
+    knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}},
+             {NPBMZ_SCHED_LU_FORWARD, {1, 0}}};
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
@@ -1415,6 +1433,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
     switch (which) {
       case NPBMZ_SCHED_LU_SWEEP: return c.lu_sweep;
       case NPBMZ_SCHED_LU_THREADS: return c.lu_nt;
+      case NPBMZ_SCHED_LU_FORWARD: return c.lu_fwd;
       case NPBMZ_SCHED_SOLVE_PAIR: return c.solve_pair;
       case NPBMZ_SCHED_SOLVE_TWIST: return c.solve_twist;
       case NPBMZ_SCHED_SOLVE_GRID: return c.solve_grid[0];
@@ -1423,7 +1442,9 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   };
   auto set = [&](int which, int v) -> int {
     if (which >= 0) {
-      if (which == NPBMZ_SCHED_LU_THREADS && c.lu_sweep != kSweepZone) return NPBMZ_OK;
+      if ((which == NPBMZ_SCHED_LU_THREADS || which == NPBMZ_SCHED_LU_FORWARD) &&
+          c.lu_sweep != kSweepZone)
+        return NPBMZ_OK;
       return npbmz_set_schedule(ctx, which, v);
     }
     c.largest_first = v == 1;
@@ -1494,6 +1515,11 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench != NPBMZ_LU || (value != 128 && value != 256 && value != 512))
         return fail(ctx, NPBMZ_EINVAL, "LU workgroup size is 128, 256 or 512");
       c.lu_nt = value;
+      break;
+    case NPBMZ_SCHED_LU_FORWARD:
+      if (c.bench != NPBMZ_LU || (value != 0 && value != 1))
+        return fail(ctx, NPBMZ_EINVAL, "LU hyperplane forwarding is 0 or 1");
+      c.lu_fwd = value;
       break;
     case NPBMZ_SCHED_SOLVE_PAIR:
       if (c.bench != NPBMZ_BT || value < 0 || value > 7)

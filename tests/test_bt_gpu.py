@@ -243,6 +243,8 @@ def test_errors_are_reported_not_raised_across_the_abi(gpu):
         ctx.set_schedule(N.SCHED_LU_SWEEP, 1)
     with pytest.raises(gpu.NpbmzError, match="two-sided"):
         ctx.set_schedule(N.SCHED_SOLVE_TWIST, 9)
+    with pytest.raises(gpu.NpbmzError, match="forwarding"):
+        ctx.set_schedule(N.SCHED_LU_FORWARD, 1)
     with pytest.raises(gpu.NpbmzError, match="unknown phase"):
         ctx.run_phase(17)
     with pytest.raises(gpu.NpbmzError, match="nsteps"):

@@ -45,7 +45,9 @@ def test_init_exact_is_bit_exact(gpu, cls):
     ctx.close()
 
 
-SCHEDULES = [("plane", 0, 256), ("zone", 1, 128), ("zone", 1, 256), ("zone", 1, 512)]
+# (name, sweep, threads, forwarding through LDS); the zone sweeps forward by default
+SCHEDULES = [("plane", 0, 256, 1), ("zone", 1, 128, 1), ("zone", 1, 256, 1), ("zone", 1, 512, 1),
+             ("zone-global", 1, 512, 0)]
 
 
 @pytest.mark.parametrize("cls", ["S", "W"])
@@ -55,11 +57,14 @@ SCHEDULES = [("plane", 0, 256), ("zone", 1, 128), ("zone", 1, 256), ("zone", 1, 
 def test_each_phase_is_bit_exact(gpu, cls, npb, mode, sched):
     """exch_qbc, the lower sweep (rsd *= dt, jacld, blts), the upper sweep
     (jacu, buts), the update and rhs, one at a time from the oracle's state a
-    few iterations into the run, under every sweep schedule."""
+    few iterations into the run, under every sweep schedule, with each
+    hyperplane of a zone sweep handed to the next through LDS or read back
+    from global memory."""
     N = gpu.native
     ctx, zones, p = _ctx(gpu, cls, True, npb)
     ctx.set_schedule(N.SCHED_LU_SWEEP, sched[1])
     ctx.set_schedule(N.SCHED_LU_THREADS, sched[2])
+    ctx.set_schedule(N.SCHED_LU_FORWARD, sched[3])
     o = _oracle(p, npb)
     o.step(3)
     _load(ctx, o, zones)
@@ -149,11 +154,13 @@ def test_schedules_and_modes_agree_class_A(gpu, strict):
     rounding."""
     N = gpu.native
     res = []
-    for sweep, nt, mode in ((0, 256, "batched"), (1, 256, "batched"), (1, 512, "graph"),
-                            (1, 128, "perzone"), (0, 256, "graph")):
+    for sweep, nt, mode, fwd in ((0, 256, "batched", 1), (1, 256, "batched", 1),
+                                 (1, 512, "graph", 1), (1, 128, "perzone", 1),
+                                 (0, 256, "graph", 1), (1, 512, "batched", 0)):
         ctx, zones, p = _ctx(gpu, "A", strict)
         ctx.set_schedule(N.SCHED_LU_SWEEP, sweep)
         ctx.set_schedule(N.SCHED_LU_THREADS, nt)
+        ctx.set_schedule(N.SCHED_LU_FORWARD, fwd)
         ctx.init_exact()
         ctx.step(4, mode)
         res.append([ctx.get_dense(z, U) for z in range(zones.count)])
