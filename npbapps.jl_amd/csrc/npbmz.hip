@@ -88,6 +88,8 @@ namespace exact {
 #pragma clang fp contract(off)
 // Reciprocals of the solves: IEEE division, as the reference computes them.
 constexpr bool kIeeeDiv = true;
+// LU's jacld/jacu blocks with the reference's own operand grouping
+constexpr bool kCollapsedBlocks = false;
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 #include "bt_kernels.inc"
 #include "sp_kernels.inc"
@@ -102,6 +104,9 @@ namespace fast {
 // dependent chain than IEEE division (scaling, reciprocal, refinement and
 // fix-up).
 constexpr bool kIeeeDiv = false;
+// LU's jacld/jacu blocks from per-point velocities and the collapsed
+// constant factors of LuK (about half the arithmetic)
+constexpr bool kCollapsedBlocks = true;
 __device__ __forceinline__ double recip(double x) {
   double r = __builtin_amdgcn_rcp(x);
   r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
@@ -274,6 +279,34 @@ LuK make_lu_constants(double dt, double omega, int nx0, int ny0, int nz0) {
   k.r43 = (4.0 / 3.0);
   k.c1345 = c1 * c3 * c4 * c5;
   k.c34 = c3 * c4;
+  const double c2 = 0.40, dd[3] = {0.75, 0.75, 1.00};  // dx1, dy1, dz1
+  const double t1[3] = {k.tx1, k.ty1, k.tz1}, t2[3] = {k.tx2, k.ty2, k.tz2};
+  for (int d = 0; d < 3; ++d) {
+    LuK::Nb& n = k.nb[d];
+    const double a = dt * t2[d], v = dt * t1[d];
+    n.a = a;
+    n.a_c2 = a * c2;
+    n.a_2mc2 = a * (2.0 - c2);
+    n.a_c1 = a * c1;
+    n.a_c2h = a * c2 * 0.5;
+    n.v_dd = v * dd[d];
+    n.v_r43c34 = v * k.r43 * k.c34;
+    n.v_c34 = v * k.c34;
+    n.v_c1345 = v * k.c1345;
+    n.v_ra = v * (k.r43 * k.c34 - k.c1345);
+    n.v_rb = v * (k.c34 - k.c1345);
+  }
+  k.d0 = 1.0 + 2.0 * dt * (t1[0] * dd[0] + t1[1] * dd[1] + t1[2] * dd[2]);
+  for (int i = 0; i < 3; ++i) {
+    double gs = 0.0, hs = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      gs += t1[d] * (d == i ? k.r43 : 1.0);
+      hs += t1[d] * (d == i ? k.r43 * k.c34 - k.c1345 : k.c34 - k.c1345);
+    }
+    k.g[i] = 2.0 * dt * k.c34 * gs;
+    k.h[i] = 2.0 * dt * hs;
+  }
+  k.h5 = 2.0 * dt * k.c1345 * (t1[0] + t1[1] + t1[2]);
   return k;
 }
 

@@ -53,6 +53,19 @@ struct LuK {
   double tx1, tx2, tx3, ty1, ty2, ty3, tz1, tz2, tz3;
   double dxi, deta, dzeta;
   double r43, c34, c1345;
+  // jacld/jacu's constant factors collapsed (FMA arithmetic only): per
+  // direction d with t1, t2 = tx1, tx2 (ty, tz) and dd = dxc (dyc, dzc),
+  // a = dt*t2 (jacu's sign; jacld's is -a), v = dt*t1:
+  struct Nb {
+    double a, a_c2, a_2mc2, a_c1, a_c2h;       // a, a*c2, a*(2-c2), a*c1, a*c2/2
+    double v_dd, v_r43c34, v_c34, v_c1345;     // v*dd, v*r43*c34, v*c34, v*c1345
+    double v_ra, v_rb;                         // v*(r43*c34-c1345), v*(c34-c1345)
+  } nb[3];
+  // the diagonal block: d0 = 1 + 2dt(tx1 dxc + ty1 dyc + tz1 dzc); g[i] =
+  // 2dt c34 (tx1 + ty1 + tz1 with r43 on direction i); h[i] = 2dt((r43 c34 -
+  // c1345) on direction i's t1 + (c34 - c1345) on the others); h5 = 2dt c1345
+  // (tx1 + ty1 + tz1)
+  double d0, g[3], h[3], h5;
 };
 
 }  // namespace npbmz
