@@ -280,7 +280,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes, "copy_kernel": copy},
+                     "algorithmic_bytes": alg_bytes, "copy_kernel": copy,
+                     # the same launch priced with its measured HBM bytes (workspace included)
+                     "measured_GBps": None if traffic is None
+                     else round(traffic / per_launch_s[dom] / 1e9, 1),
+                     "measured_frac": None if traffic is None
+                     else round(traffic / per_launch_s[dom] / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
