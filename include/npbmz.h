@@ -182,11 +182,12 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                      lines than the device has CUs); the
                                      adjoint's transposed solves follow the
                                      same bits */
-#define NPBMZ_SCHED_LU_FORWARD 5 /* LU workgroup-per-zone sweeps: 1 (default) hands
-                                    each hyperplane's new rsd to the next through
-                                    LDS when two hyperplanes of a zone fit in 64
-                                    KiB, 0 reads it back from global memory (same
-                                    values) */
+#define NPBMZ_SCHED_LU_FORWARD 5 /* LU workgroup-per-zone sweeps: each hyperplane
+                                    hands the next its new rsd (1), or its rsd and
+                                    u (2, default), through LDS, as far as two
+                                    hyperplanes fit the device's LDS per
+                                    workgroup; 0 reads them from global memory
+                                    (same values) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -131,7 +131,6 @@ constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
-constexpr size_t kLuFwdMaxLds = 65536;  // LDS a zone sweep may take to forward hyperplanes
 
 // LU sweep schedules: one launch per hyperplane over all zones, or one
 // workgroup per zone walking all hyperplanes of the sweep.
@@ -172,7 +171,8 @@ struct npbmz_ctx {
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
-  int lu_fwd = 1;                 // per-zone sweep: forward each hyperplane through LDS
+  int lu_fwd = 2;                 // per-zone sweep: forward each hyperplane's rsd (1) and u (2) through LDS
+  size_t max_lds = 65536;         // dynamic LDS a zone-sweep workgroup may take
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Face* faces = nullptr;
   double* partial = nullptr;
@@ -578,27 +578,34 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
   const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
   const int ns = NI + NJ + NK - 2;
   if (c.lu_sweep == kSweepZone) {
-    // the hyperplane forwarding through LDS when two hyperplanes fit
-    const size_t lds = sizeof(double) * 2 * 5 * (size_t)NJ * NK;
-    const bool fwd = c.lu_fwd && lds <= kLuFwdMaxLds;
+    // hyperplane forwarding through LDS: rsd and u (2) or rsd only (1), as
+    // far as the two hyperplanes' buffers fit the workgroup's LDS
+    const size_t lds1 = sizeof(double) * 2 * 5 * (size_t)NJ * NK, lds2 = 2 * lds1;
+    int lvl = c.lu_fwd;
+    if (lvl >= 2 && lds2 > c.max_lds) lvl = 1;
+    if (lvl >= 1 && lds1 > c.max_lds) lvl = 0;
+    const size_t lds = lvl == 2 ? lds2 : lvl == 1 ? lds1 : 0;
     auto go = [&](auto kern, int nt) {
-      hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), fwd ? lds : 0, c.stream, c.u, c.rhs, c.zd, z0,
-                         c.lk);
+      hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), lds, c.stream, c.u, c.rhs, c.zd, z0, c.lk);
     };
-    auto pick = [&](auto kfwd, auto kglobal, int nt) { fwd ? go(kfwd, nt) : go(kglobal, nt); };
+    auto pick = [&](auto k0, auto k1, auto k2, int nt) {
+      if (lvl == 2) go(k2, nt);
+      else if (lvl == 1) go(k1, nt);
+      else go(k0, nt);
+    };
     const bool s = strict(ctx);
     switch (c.lu_nt) {
       case 128:
-        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 128, true>, exact::lu::sweep_zone_kernel<UPPER, 128, false>, 128)
-          : pick(fast::lu::sweep_zone_kernel<UPPER, 128, true>, fast::lu::sweep_zone_kernel<UPPER, 128, false>, 128);
+        if (s) pick(exact::lu::sweep_zone_kernel<UPPER, 128, 0>, exact::lu::sweep_zone_kernel<UPPER, 128, 1>, exact::lu::sweep_zone_kernel<UPPER, 128, 2>, 128);
+        else pick(fast::lu::sweep_zone_kernel<UPPER, 128, 0>, fast::lu::sweep_zone_kernel<UPPER, 128, 1>, fast::lu::sweep_zone_kernel<UPPER, 128, 2>, 128);
         break;
       case 512:
-        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 512, true>, exact::lu::sweep_zone_kernel<UPPER, 512, false>, 512)
-          : pick(fast::lu::sweep_zone_kernel<UPPER, 512, true>, fast::lu::sweep_zone_kernel<UPPER, 512, false>, 512);
+        if (s) pick(exact::lu::sweep_zone_kernel<UPPER, 512, 0>, exact::lu::sweep_zone_kernel<UPPER, 512, 1>, exact::lu::sweep_zone_kernel<UPPER, 512, 2>, 512);
+        else pick(fast::lu::sweep_zone_kernel<UPPER, 512, 0>, fast::lu::sweep_zone_kernel<UPPER, 512, 1>, fast::lu::sweep_zone_kernel<UPPER, 512, 2>, 512);
         break;
       default:
-        s ? pick(exact::lu::sweep_zone_kernel<UPPER, 256, true>, exact::lu::sweep_zone_kernel<UPPER, 256, false>, 256)
-          : pick(fast::lu::sweep_zone_kernel<UPPER, 256, true>, fast::lu::sweep_zone_kernel<UPPER, 256, false>, 256);
+        if (s) pick(exact::lu::sweep_zone_kernel<UPPER, 256, 0>, exact::lu::sweep_zone_kernel<UPPER, 256, 1>, exact::lu::sweep_zone_kernel<UPPER, 256, 2>, 256);
+        else pick(fast::lu::sweep_zone_kernel<UPPER, 256, 0>, fast::lu::sweep_zone_kernel<UPPER, 256, 1>, fast::lu::sweep_zone_kernel<UPPER, 256, 2>, 256);
         break;
     }
     CK(hipGetLastError());
@@ -965,7 +972,10 @@ void describe(npbmz_ctx* c) {
   if (c->bench == NPBMZ_LU)
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
-             c->lu_sweep == kSweepZone && c->lu_fwd ? " lds-forward" : "",
+             c->lu_sweep != kSweepZone ? ""
+             : c->lu_fwd == 2          ? " lds-forward=rsd+u"
+             : c->lu_fwd == 1          ? " lds-forward=rsd"
+                                       : "",
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else if (c->bench == NPBMZ_SP)
     snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d twist=%d fp=%s", c->nz_,
@@ -1114,6 +1124,34 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   CKC(hipMemset(c.frc, 0, fb));
   CKC(hipMemset(c.rhs, 0, fb));
   if ((rc = build_tables(ctx))) return bad(rc);
+  if (c.bench == NPBMZ_LU) {
+    // the zone sweeps' LDS: the device's per-workgroup maximum, granted to
+    // every instantiation that forwards hyperplanes (64 KiB if refused)
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 65536) {
+      bool ok = true;
+      auto grant = [&](auto kern) {
+        ok = ok && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess;
+      };
+      grant(exact::lu::sweep_zone_kernel<false, 128, 1>); grant(exact::lu::sweep_zone_kernel<true, 128, 1>);
+      grant(exact::lu::sweep_zone_kernel<false, 256, 1>); grant(exact::lu::sweep_zone_kernel<true, 256, 1>);
+      grant(exact::lu::sweep_zone_kernel<false, 512, 1>); grant(exact::lu::sweep_zone_kernel<true, 512, 1>);
+      grant(exact::lu::sweep_zone_kernel<false, 128, 2>); grant(exact::lu::sweep_zone_kernel<true, 128, 2>);
+      grant(exact::lu::sweep_zone_kernel<false, 256, 2>); grant(exact::lu::sweep_zone_kernel<true, 256, 2>);
+      grant(exact::lu::sweep_zone_kernel<false, 512, 2>); grant(exact::lu::sweep_zone_kernel<true, 512, 2>);
+      grant(fast::lu::sweep_zone_kernel<false, 128, 1>); grant(fast::lu::sweep_zone_kernel<true, 128, 1>);
+      grant(fast::lu::sweep_zone_kernel<false, 256, 1>); grant(fast::lu::sweep_zone_kernel<true, 256, 1>);
+      grant(fast::lu::sweep_zone_kernel<false, 512, 1>); grant(fast::lu::sweep_zone_kernel<true, 512, 1>);
+      grant(fast::lu::sweep_zone_kernel<false, 128, 2>); grant(fast::lu::sweep_zone_kernel<true, 128, 2>);
+      grant(fast::lu::sweep_zone_kernel<false, 256, 2>); grant(fast::lu::sweep_zone_kernel<true, 256, 2>);
+      grant(fast::lu::sweep_zone_kernel<false, 512, 2>); grant(fast::lu::sweep_zone_kernel<true, 512, 2>);
+      (void)hipGetLastError();
+      if (ok) c.max_lds = (size_t)v;
+    }
+  }
   if (c.bench == NPBMZ_SP) {
     // default: two-sided solves in the directions with fewer line tasks
     // than the chip has CUs, as measured on MI355X (class A, at most 225
@@ -1455,7 +1493,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   std::vector<Knob> knobs;
   if (c.bench == NPBMZ_LU) {
     knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}},
-             {NPBMZ_SCHED_LU_FORWARD, {1, 0}}};
+             {NPBMZ_SCHED_LU_FORWARD, {2, 1, 0}}};
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
@@ -1550,8 +1588,8 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       c.lu_nt = value;
       break;
     case NPBMZ_SCHED_LU_FORWARD:
-      if (c.bench != NPBMZ_LU || (value != 0 && value != 1))
-        return fail(ctx, NPBMZ_EINVAL, "LU hyperplane forwarding is 0 or 1");
+      if (c.bench != NPBMZ_LU || value < 0 || value > 2)
+        return fail(ctx, NPBMZ_EINVAL, "LU hyperplane forwarding is 0, 1 or 2");
       c.lu_fwd = value;
       break;
     case NPBMZ_SCHED_SOLVE_PAIR:

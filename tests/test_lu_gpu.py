@@ -45,9 +45,9 @@ def test_init_exact_is_bit_exact(gpu, cls):
     ctx.close()
 
 
-# (name, sweep, threads, forwarding through LDS); the zone sweeps forward by default
-SCHEDULES = [("plane", 0, 256, 1), ("zone", 1, 128, 1), ("zone", 1, 256, 1), ("zone", 1, 512, 1),
-             ("zone-global", 1, 512, 0)]
+# (name, sweep, threads, forwarding through LDS: 0 none, 1 rsd, 2 rsd and u, the default)
+SCHEDULES = [("plane", 0, 256, 2), ("zone", 1, 128, 2), ("zone", 1, 256, 2), ("zone", 1, 512, 2),
+             ("zone-rsd-lds", 1, 512, 1), ("zone-global", 1, 512, 0)]
 
 
 @pytest.mark.parametrize("cls", ["S", "W"])
@@ -154,9 +154,10 @@ def test_schedules_and_modes_agree_class_A(gpu, strict):
     rounding."""
     N = gpu.native
     res = []
-    for sweep, nt, mode, fwd in ((0, 256, "batched", 1), (1, 256, "batched", 1),
-                                 (1, 512, "graph", 1), (1, 128, "perzone", 1),
-                                 (0, 256, "graph", 1), (1, 512, "batched", 0)):
+    for sweep, nt, mode, fwd in ((0, 256, "batched", 2), (1, 256, "batched", 2),
+                                 (1, 512, "graph", 2), (1, 128, "perzone", 2),
+                                 (0, 256, "graph", 2), (1, 512, "batched", 0),
+                                 (1, 512, "batched", 1)):
         ctx, zones, p = _ctx(gpu, "A", strict)
         ctx.set_schedule(N.SCHED_LU_SWEEP, sweep)
         ctx.set_schedule(N.SCHED_LU_THREADS, nt)
