@@ -569,6 +569,20 @@ int k_lu_update(npbmz_ctx* ctx, int zone) {
   return NPBMZ_OK;
 }
 
+// The zone sweeps' hyperplane forwarding through LDS: the requested level
+// (2 rsd and u, 1 rsd only, 0 none) lowered until the two hyperplanes'
+// buffers fit the workgroup's LDS; and the dynamic LDS it takes.
+size_t lu_fwd_lds(const npbmz_ctx* c, int lvl) {
+  const size_t one = sizeof(double) * 2 * 5 * (size_t)(c->ny[0] - 2) * (c->nz[0] - 2);
+  return lvl == 2 ? 2 * one : lvl == 1 ? one : 0;
+}
+
+int lu_fwd_level(const npbmz_ctx* c) {
+  int lvl = c->lu_fwd;
+  while (lvl > 0 && lu_fwd_lds(c, lvl) > c->max_lds) --lvl;
+  return lvl;
+}
+
 // One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
 // zone (zone >= 0) or all zones, in the context's schedule.
 template <bool UPPER>
@@ -578,13 +592,8 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
   const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
   const int ns = NI + NJ + NK - 2;
   if (c.lu_sweep == kSweepZone) {
-    // hyperplane forwarding through LDS: rsd and u (2) or rsd only (1), as
-    // far as the two hyperplanes' buffers fit the workgroup's LDS
-    const size_t lds1 = sizeof(double) * 2 * 5 * (size_t)NJ * NK, lds2 = 2 * lds1;
-    int lvl = c.lu_fwd;
-    if (lvl >= 2 && lds2 > c.max_lds) lvl = 1;
-    if (lvl >= 1 && lds1 > c.max_lds) lvl = 0;
-    const size_t lds = lvl == 2 ? lds2 : lvl == 1 ? lds1 : 0;
+    const int lvl = lu_fwd_level(&c);
+    const size_t lds = lu_fwd_lds(&c, lvl);
     auto go = [&](auto kern, int nt) {
       hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), lds, c.stream, c.u, c.rhs, c.zd, z0, c.lk);
     };
@@ -973,8 +982,8 @@ void describe(npbmz_ctx* c) {
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              c->lu_sweep != kSweepZone ? ""
-             : c->lu_fwd == 2          ? " lds-forward=rsd+u"
-             : c->lu_fwd == 1          ? " lds-forward=rsd"
+             : lu_fwd_level(c) == 2    ? " lds-forward=rsd+u"
+             : lu_fwd_level(c) == 1    ? " lds-forward=rsd"
                                        : "",
              (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference", strict(c) ? "strict" : "fma");
   else if (c->bench == NPBMZ_SP)
