@@ -184,10 +184,15 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                      same bits */
 #define NPBMZ_SCHED_LU_FORWARD 5 /* LU workgroup-per-zone sweeps: each hyperplane
                                     hands the next its new rsd (1), or its rsd and
-                                    u (2, default), through LDS, as far as two
-                                    hyperplanes fit the device's LDS per
-                                    workgroup; 0 reads them from global memory
-                                    (same values) */
+                                    u (2), through LDS, as far as two hyperplanes
+                                    fit the device's LDS per workgroup; 0 reads
+                                    them from global memory (same values).  3
+                                    (default): with FMA arithmetic and at most
+                                    one (j, k) slot of a hyperplane padded by a
+                                    halo row and column per thread, the zone's
+                                    boundary values pass through LDS too and each
+                                    thread's global loads are issued a hyperplane
+                                    ahead; otherwise as 2 */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

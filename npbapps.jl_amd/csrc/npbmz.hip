@@ -171,7 +171,8 @@ struct npbmz_ctx {
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
-  int lu_fwd = 2;                 // per-zone sweep: forward each hyperplane's rsd (1) and u (2) through LDS
+  int lu_fwd = 3;                 // per-zone sweep: forward each hyperplane's rsd (1) and u (2)
+                                  // through LDS; 3: the padded-halo sweep where it applies
   size_t max_lds = 65536;         // dynamic LDS a zone-sweep workgroup may take
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Face* faces = nullptr;
@@ -578,9 +579,20 @@ size_t lu_fwd_lds(const npbmz_ctx* c, int lvl) {
 }
 
 int lu_fwd_level(const npbmz_ctx* c) {
-  int lvl = c->lu_fwd;
+  int lvl = std::min(c->lu_fwd, 2);
   while (lvl > 0 && lu_fwd_lds(c, lvl) > c->max_lds) --lvl;
   return lvl;
+}
+
+// Level 3, the padded-halo sweep with loads a hyperplane ahead: FMA
+// arithmetic, one padded (j, k) slot per thread, its LDS within the device's.
+size_t lu_halo_lds(const npbmz_ctx* c) {
+  return sizeof(double) * 2 * 10 * (size_t)(c->ny[0] - 1) * (c->nz[0] - 1);
+}
+
+bool lu_halo_ok(const npbmz_ctx* c) {
+  return c->lu_fwd == 3 && !(c->flags & NPBMZ_FLAG_STRICT_FP) &&
+         (c->ny[0] - 1) * (c->nz[0] - 1) <= c->lu_nt && lu_halo_lds(c) <= c->max_lds;
 }
 
 // One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
@@ -603,6 +615,19 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
       else go(k0, nt);
     };
     const bool s = strict(ctx);
+    if (lu_halo_ok(&c)) {
+      auto goh = [&](auto kern, int nt) {
+        hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), lu_halo_lds(&c), c.stream, c.u, c.rhs, c.zd,
+                           z0, c.lk);
+      };
+      switch (c.lu_nt) {
+        case 128: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 128>, 128); break;
+        case 512: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 512>, 512); break;
+        default: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 256>, 256); break;
+      }
+      CK(hipGetLastError());
+      return NPBMZ_OK;
+    }
     switch (c.lu_nt) {
       case 128:
         if (s) pick(exact::lu::sweep_zone_kernel<UPPER, 128, 0>, exact::lu::sweep_zone_kernel<UPPER, 128, 1>, exact::lu::sweep_zone_kernel<UPPER, 128, 2>, 128);
@@ -982,6 +1007,7 @@ void describe(npbmz_ctx* c) {
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              c->lu_sweep != kSweepZone ? ""
+             : lu_halo_ok(c)           ? " lds-halo+prefetch"
              : lu_fwd_level(c) == 2    ? " lds-forward=rsd+u"
              : lu_fwd_level(c) == 1    ? " lds-forward=rsd"
                                        : "",
@@ -1157,6 +1183,9 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       grant(fast::lu::sweep_zone_kernel<false, 128, 2>); grant(fast::lu::sweep_zone_kernel<true, 128, 2>);
       grant(fast::lu::sweep_zone_kernel<false, 256, 2>); grant(fast::lu::sweep_zone_kernel<true, 256, 2>);
       grant(fast::lu::sweep_zone_kernel<false, 512, 2>); grant(fast::lu::sweep_zone_kernel<true, 512, 2>);
+      grant(fast::lu::sweep_zone_halo_kernel<false, 128>); grant(fast::lu::sweep_zone_halo_kernel<true, 128>);
+      grant(fast::lu::sweep_zone_halo_kernel<false, 256>); grant(fast::lu::sweep_zone_halo_kernel<true, 256>);
+      grant(fast::lu::sweep_zone_halo_kernel<false, 512>); grant(fast::lu::sweep_zone_halo_kernel<true, 512>);
       (void)hipGetLastError();
       if (ok) c.max_lds = (size_t)v;
     }
@@ -1502,7 +1531,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   std::vector<Knob> knobs;
   if (c.bench == NPBMZ_LU) {
     knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}},
-             {NPBMZ_SCHED_LU_FORWARD, {2, 1, 0}}};
+             {NPBMZ_SCHED_LU_FORWARD, {3, 2, 1, 0}}};
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
@@ -1597,8 +1626,8 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       c.lu_nt = value;
       break;
     case NPBMZ_SCHED_LU_FORWARD:
-      if (c.bench != NPBMZ_LU || value < 0 || value > 2)
-        return fail(ctx, NPBMZ_EINVAL, "LU hyperplane forwarding is 0, 1 or 2");
+      if (c.bench != NPBMZ_LU || value < 0 || value > 3)
+        return fail(ctx, NPBMZ_EINVAL, "LU hyperplane forwarding is 0, 1, 2 or 3");
       c.lu_fwd = value;
       break;
     case NPBMZ_SCHED_SOLVE_PAIR:

@@ -45,9 +45,10 @@ def test_init_exact_is_bit_exact(gpu, cls):
     ctx.close()
 
 
-# (name, sweep, threads, forwarding through LDS: 0 none, 1 rsd, 2 rsd and u, the default)
-SCHEDULES = [("plane", 0, 256, 2), ("zone", 1, 128, 2), ("zone", 1, 256, 2), ("zone", 1, 512, 2),
-             ("zone-rsd-lds", 1, 512, 1), ("zone-global", 1, 512, 0)]
+# (name, sweep, threads, forwarding through LDS: 0 none, 1 rsd, 2 rsd and u, 3 (the default)
+# the padded-halo sweep, which strict arithmetic runs as 2)
+SCHEDULES = [("plane", 0, 256, 3), ("zone", 1, 128, 3), ("zone", 1, 256, 3), ("zone", 1, 512, 3),
+             ("zone-rsd-u-lds", 1, 512, 2), ("zone-rsd-lds", 1, 512, 1), ("zone-global", 1, 512, 0)]
 
 
 @pytest.mark.parametrize("cls", ["S", "W"])
@@ -154,10 +155,10 @@ def test_schedules_and_modes_agree_class_A(gpu, strict):
     rounding."""
     N = gpu.native
     res = []
-    for sweep, nt, mode, fwd in ((0, 256, "batched", 2), (1, 256, "batched", 2),
-                                 (1, 512, "graph", 2), (1, 128, "perzone", 2),
-                                 (0, 256, "graph", 2), (1, 512, "batched", 0),
-                                 (1, 512, "batched", 1)):
+    for sweep, nt, mode, fwd in ((0, 256, "batched", 3), (1, 256, "batched", 3),
+                                 (1, 512, "graph", 3), (1, 128, "perzone", 3),
+                                 (0, 256, "graph", 3), (1, 512, "batched", 0),
+                                 (1, 512, "batched", 1), (1, 512, "batched", 2)):
         ctx, zones, p = _ctx(gpu, "A", strict)
         ctx.set_schedule(N.SCHED_LU_SWEEP, sweep)
         ctx.set_schedule(N.SCHED_LU_THREADS, nt)
