@@ -879,6 +879,23 @@ template <bool ASC>
 int k_lu_adj_sweep(npbmz_ctx* ctx, const double* ue, const double* ubar, double* nu, double* rho) {
   auto& c = *ctx;
   if (c.lu_sweep == kSweepZone) {
+    const int np = (c.ny[0] - 2) * (c.nz[0] - 2);
+    if (c.lu_fwd == 3 && np <= c.lu_nt) {
+      // the forward's level 3: hyperplanes handed on through LDS, own data
+      // loaded a hyperplane ahead (2*5*np doubles, within 64 KiB for np <= 512)
+      const size_t lds = sizeof(double) * 2 * 5 * (size_t)np;
+      auto gof = [&](auto kern, int nt) {
+        hipLaunchKernelGGL(kern, dim3(c.nz_), dim3(nt), lds, c.stream, ue, ubar, nu, rho, c.zd,
+                           c.lk);
+      };
+      switch (c.lu_nt) {
+        case 128: gof(fast::lu::adj_sweep_zone_fwd_kernel<ASC, 128>, 128); break;
+        case 512: gof(fast::lu::adj_sweep_zone_fwd_kernel<ASC, 512>, 512); break;
+        default: gof(fast::lu::adj_sweep_zone_fwd_kernel<ASC, 256>, 256); break;
+      }
+      CK(hipGetLastError());
+      return NPBMZ_OK;
+    }
     auto go = [&](auto kern, int nt) {
       hipLaunchKernelGGL(kern, dim3(c.nz_), dim3(nt), 0, c.stream, ue, ubar, nu, rho, c.zd, c.lk);
     };

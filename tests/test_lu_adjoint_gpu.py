@@ -26,20 +26,24 @@ def _rng(seed):
                                                        ("S", 6, 20, False, None),
                                                        ("W", 6, 4, True, None),
                                                        ("S", 5, 2, True, (0, 256)),
-                                                       ("S", 5, 2, True, (1, 128))])
+                                                       ("S", 5, 2, True, (1, 128)),
+                                                       ("S", 5, 2, True, (1, 512, 0))])
 def test_adjoint_matches_dual_numbers(gpu, cls, nsteps, ckpt, npb, sweep):
     """Reverse mode on the GPU against forward mode on the CPU oracle: the
     directional derivatives along 8 random (u0, frct) directions agree to
     1e-9 and the loss to 1e-12; u is back at u0 afterwards.  sweep: the LU
     sweep schedule (hyperplane launches, or a workgroup per zone of the given
-    size) that both the forward and the transposed sweeps follow; None keeps
-    the default."""
+    size, and optionally its forwarding level) that both the forward and the
+    transposed sweeps follow; None keeps the default (a workgroup per zone
+    handing hyperplanes on through LDS and loading a hyperplane ahead)."""
     N = gpu.native
     zones, p = gpu.zones_for("LU", cls)
     ctx = gpu.Context("LU", zones, p.dt, omega=1.2, npb_exchange=npb)
     if sweep is not None:
         ctx.set_schedule(N.SCHED_LU_SWEEP, sweep[0])
         ctx.set_schedule(N.SCHED_LU_THREADS, sweep[1])
+        if len(sweep) > 2:
+            ctx.set_schedule(N.SCHED_LU_FORWARD, sweep[2])
     ctx.init_exact()
     n = ctx.size
     r = _rng(2026)
