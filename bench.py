@@ -291,6 +291,9 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds, threads)
+        # the same restatement on one core, for the per-core figure
+        one = cpu_baseline(p, args.cpu_seconds, 1)
+        out["cpu_baseline"]["single_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
     print(json.dumps(out))
 
 
