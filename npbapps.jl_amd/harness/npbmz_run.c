@@ -8,10 +8,12 @@
  * src/LU/zone_setup.jl:15,19), the class parameters (src/BT/classes.jl:95-104,
  * src/SP/classes.jl:14-24, src/LU/classes.jl:14-24), initialisation, one
  * untimed step followed by re-initialisation (src/BT/bt-node.jl:138-253),
- * niter timed steps, and the node-side verification numerics.  It prints
- * one JSON line: the norms, the time and Mop/s with the reference's flop
- * counts (src/BT/bt-driver.jl:131-140, src/LU/lu-driver.jl:142-151).  The
- * class tables themselves are checked by the caller.
+ * niter timed steps, the node-side verification numerics and the class
+ * verification (src/BT/verify.jl:355-400, src/LU/verify.jl:355-420: dt and
+ * niter as tabulated, every norm within relative 1e-8 of the table).  It
+ * prints one JSON line: the norms, the verdict, the time and Mop/s with the
+ * reference's flop counts (src/BT/bt-driver.jl:131-140,
+ * src/LU/lu-driver.jl:142-151).
  *
  *   usage: npbmz_run bt|sp|lu S|W|A|B [perzone|batched|graph] [strict] [npbexch]
  *          npbmz_run bt|sp|lu S|W|A|B zones   (the zone table only, no GPU)
@@ -22,6 +24,7 @@
 #include <string.h>
 
 #include "npbmz.h"
+#include "verify_tables.h"
 
 typedef struct {
   const char* name;
@@ -77,6 +80,23 @@ static double mflops_per_step(int bench, int n, const int32_t* nx, const int32_t
     tot += 1.0e-6 * (a * n3 - b * nsur + c * navg - d);
   }
   return tot;
+}
+
+/* The class table's verdict on a run (1 verified, 0 not, -1 no table). */
+static int verified(const char* bench, const char* cls, double dt, int niter, const double xcr[5],
+                    const double xce[5], double xci) {
+  for (size_t t = 0; t < sizeof npbmz_vtables / sizeof npbmz_vtables[0]; ++t) {
+    const npbmz_vtable* v = &npbmz_vtables[t];
+    if (strcmp(v->bench, bench) || strcmp(v->cls, cls)) continue;
+    int ok = fabs(dt - v->dt) <= npbmz_veps && niter == v->niter;
+    for (int m = 0; m < 5; ++m) {
+      ok = ok && fabs((xcr[m] - v->xcr[m]) / v->xcr[m]) <= npbmz_veps;
+      ok = ok && fabs((xce[m] - v->xce[m]) / v->xce[m]) <= npbmz_veps;
+    }
+    if (v->has_xci) ok = ok && fabs((xci - v->xci) / v->xci) <= npbmz_veps;
+    return ok;
+  }
+  return -1;
 }
 
 static int check(int rc, npbmz_ctx* ctx, const char* what) {
@@ -158,10 +178,12 @@ int main(int argc, char** argv) {
   npbmz_destroy(ctx);
 
   const double mops = mflops_per_step(bench, n, nx, ny, nz) * k->niter / (ms * 1e-3);
+  const char* bname = bench == NPBMZ_BT ? "BT" : bench == NPBMZ_SP ? "SP" : "LU";
+  const int ok = verified(bname, k->name, k->dt, k->niter, xcr, xce, xci);
   printf("{\"bench\": \"%s\", \"class\": \"%s\", \"niter\": %d, \"dt\": %.17g, \"ms\": %.6f, "
          "\"mops\": %.3f, \"xcr\": [%.17g, %.17g, %.17g, %.17g, %.17g], "
-         "\"xce\": [%.17g, %.17g, %.17g, %.17g, %.17g], \"xci\": %.17g}\n",
-         bench == NPBMZ_BT ? "BT" : bench == NPBMZ_SP ? "SP" : "LU", k->name, k->niter, k->dt, ms,
-         mops, xcr[0], xcr[1], xcr[2], xcr[3], xcr[4], xce[0], xce[1], xce[2], xce[3], xce[4], xci);
+         "\"xce\": [%.17g, %.17g, %.17g, %.17g, %.17g], \"xci\": %.17g, \"verified\": %s}\n",
+         bname, k->name, k->niter, k->dt, ms, mops, xcr[0], xcr[1], xcr[2], xcr[3], xcr[4], xce[0],
+         xce[1], xce[2], xce[3], xce[4], xci, ok < 0 ? "null" : ok ? "true" : "false");
   return 0;
 }

@@ -92,6 +92,19 @@ def test_header_is_plain_c_and_the_c_harness_compiles(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+def test_harness_verification_tables_match_verify_py(tmp_path):
+    """The C harness's class tables (harness/verify_tables.h, committed) are
+    exactly what the build generates from verify.py."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "_npbmz_build", os.path.join(ROOT, "npbapps.jl_amd", "build.py"))
+    build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(build)
+    out = build.write_verify_tables(str(tmp_path / "t.h"))
+    committed = os.path.join(ROOT, "npbapps.jl_amd", "harness", "verify_tables.h")
+    assert open(out).read() == open(committed).read()
+
+
 @pytest.mark.parametrize("bench", ["bt", "sp", "lu"])
 def test_c_harness_zone_split_matches_the_python_one(bench):
     """The C caller's zone split (src/BT/zone_setup.jl, LU rounding from

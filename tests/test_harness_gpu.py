@@ -1,7 +1,8 @@
 """The C caller of the ABI (npbapps.jl_amd/harness/npbmz_run.c) on the GPU:
 whole class runs through the C entry points alone verify against the
-reference's class tables and reproduce the Python binding's norms bit for
-bit (same library, same launches)."""
+reference's class tables (by the harness's own check and by verify.py) and
+reproduce the Python binding's norms bit for bit (same library, same
+launches)."""
 import json
 import os
 import subprocess
@@ -29,6 +30,7 @@ def test_c_harness_runs_a_class_and_verifies(gpu, bench, cls, opts):
     ok, rel = gpu.verify.verify(B, cls, p.dt, p.niter, got["xcr"], got["xce"],
                                 got["xci"] if B == "LU" else None)
     assert ok, rel
+    assert got["verified"] is True  # the harness's own check of the class table
     mode = "perzone" if "perzone" in opts else "graph"
     ref = gpu.go(B, cls, mode=mode, verbose=False, strict="strict" in opts,
                  npb_exchange="npbexch" in opts)
