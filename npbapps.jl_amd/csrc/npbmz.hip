@@ -798,12 +798,11 @@ int sp_step_store(npbmz_ctx* ctx, double* us, double* r0, double* x1, double* x2
   return k_solve(ctx, 2, true, -1, x3);
 }
 
-// The transposed solve of direction DIR and its matrices' derivative; for
-// DIR 0 (given r0 and lamout) also txinvr's adjoint, into lamout.
+// The transposed solve of direction DIR: lam = A^-T (T^T gin), tzetar's
+// u-term (DIR 2) added to ubar.
 template <int DIR>
 int k_sp_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const double* gin,
-                   double* lam, double* ubar, const double* r0 = nullptr,
-                   double* lamout = nullptr) {
+                   double* lam, double* ubar) {
   auto& c = *ctx;
   const Table& t = c.t_line[DIR];
   if (t.n <= 0) return NPBMZ_OK;
@@ -817,27 +816,31 @@ int k_sp_adj_solve(npbmz_ctx* ctx, const double* us, const double* x, const doub
     hipLaunchKernelGGL(fast::sp::adj_solve_kernel<DIR>, dim3(grid), dim3(kLineNT), 0, c.stream,
                        us, x, gin, lam, ubar, c.cscr, c.zd, t.d, t.n, c.maxrows[DIR], c.K);
   CK(hipGetLastError());
-  const Table& ti = c.t_int;
-  if (ti.n > 0)
-    hipLaunchKernelGGL((fast::sp::adj_deriv_kernel<DIR, kPtNT>), dim3(ti.n), dim3(kPtNT), 0,
-                       c.stream, us, x, lam, ubar, c.zd, ti.d, c.K, r0, lamout);
-  CK(hipGetLastError());
   return NPBMZ_OK;
 }
 
 // Reverse one SP step from its stored forward data; ubar holds dL/du after
 // the step on entry and dL/du before it (before the exchange) on return;
-// fbar accumulates dL/dforcing; lam is workspace.
+// fbar accumulates dL/dforcing; lz, ly, lx (each direction's transposed
+// solution) are workspace.  The solves leave ubar alone apart from tzetar's
+// u-term, so the three matrices' derivatives and txinvr's adjoint follow in
+// one pass after the last solve.
 int sp_adj_step(npbmz_ctx* ctx, const double* us, const double* r0, const double* x1,
-                const double* x2, const double* x3, double* ubar, double* lam, double* fbar) {
+                const double* x2, const double* x3, double* ubar, double* lz, double* ly,
+                double* lx, double* fbar) {
   auto& c = *ctx;
   int rc;
-  if ((rc = k_sp_adj_solve<2>(ctx, us, x3, ubar, lam, ubar))) return rc;
-  if ((rc = k_sp_adj_solve<1>(ctx, us, x2, lam, lam, ubar))) return rc;
+  if ((rc = k_sp_adj_solve<2>(ctx, us, x3, ubar, lz, ubar))) return rc;
+  if ((rc = k_sp_adj_solve<1>(ctx, us, x2, lz, ly, ubar))) return rc;
+  if ((rc = k_sp_adj_solve<0>(ctx, us, x1, ly, lx, ubar))) return rc;
   // txinvr's adjoint goes to the step workspace rhs, free during the reverse
   // pass (off the interior it is never read: adj_rhs_kernel reads the
   // interior only)
-  if ((rc = k_sp_adj_solve<0>(ctx, us, x1, lam, lam, ubar, r0, c.rhs))) return rc;
+  const Table& ti = c.t_int;
+  if (ti.n > 0)
+    hipLaunchKernelGGL(fast::sp::adj_deriv_kernel<kPtNT>, dim3(ti.n), dim3(kPtNT), 0, c.stream,
+                       us, x1, x2, x3, lx, ly, lz, r0, c.rhs, ubar, c.zd, ti.d, c.K);
+  CK(hipGetLastError());
   const Table& t = c.t_all;
   hipLaunchKernelGGL(fast::bt::adj_rhs_kernel<kPtNT>, dim3(t.n), dim3(kPtNT), 0, c.stream, us,
                      c.rhs, ubar, fbar, c.zd, t.d, c.K);
@@ -1726,8 +1729,10 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   };
   double *wd, *ubar, *lam, *fbar, *aux = nullptr, *rs0 = nullptr;
   std::vector<double*> ck(ncp), us(k), x1(k), x2(k), x3(k), r0(k, nullptr);
+  // LU: a workspace field and the saved rsd; SP: the y and x transposed
+  // solutions (lam holds z's)
   bool ok = alloc(&wd) && alloc(&ubar) && alloc(&lam) && alloc(&fbar) &&
-            (!lu || (alloc(&aux) && alloc(&rs0)));
+            (!(lu || sp) || (alloc(&aux) && alloc(&rs0)));
   for (int q = 0; q < ncp && ok; ++q) ok = alloc(&ck[q]);
   for (int q = 0; q < k && ok; ++q) ok = alloc(&us[q]) && alloc(&x1[q]) && alloc(&x2[q]) && alloc(&x3[q]);
   for (int q = 0; q < k && ok && sp; ++q) ok = alloc(&r0[q]);
@@ -1773,7 +1778,7 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
     }
     for (int n = n1 - 1; n >= n0 && rc == NPBMZ_OK; --n) {
       const int q = n - n0;
-      rc = sp   ? sp_adj_step(ctx, us[q], r0[q], x1[q], x2[q], x3[q], ubar, lam, fbar)
+      rc = sp   ? sp_adj_step(ctx, us[q], r0[q], x1[q], x2[q], x3[q], ubar, lam, aux, rs0, fbar)
            : lu ? lu_adj_step(ctx, x3[q], us[q], x1[q], x2[q], ubar, lam, aux, fbar)
                 : bt_adj_step(ctx, us[q], x1[q], x2[q], x3[q], ubar, lam, fbar);
     }
