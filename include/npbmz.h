@@ -186,13 +186,14 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     hands the next its new rsd (1), or its rsd and
                                     u (2), through LDS, as far as two hyperplanes
                                     fit the device's LDS per workgroup; 0 reads
-                                    them from global memory (same values).  3
-                                    (default): with FMA arithmetic and at most
-                                    one (j, k) slot of a hyperplane padded by a
-                                    halo row and column per thread, the zone's
-                                    boundary values pass through LDS too and each
-                                    thread's global loads are issued a hyperplane
-                                    ahead; otherwise as 2 */
+                                    them from global memory (same values).  3:
+                                    with FMA arithmetic and at most one (j, k)
+                                    slot of a hyperplane padded by a halo row and
+                                    column per thread, the zone's boundary values
+                                    pass through LDS too and each thread's global
+                                    loads are issued a hyperplane ahead;
+                                    otherwise as 2.  Default: 3 from 128 (j, k)
+                                    pairs per hyperplane up, else 0 */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -27,15 +27,18 @@ def _rng(seed):
                                                        ("W", 6, 4, True, None),
                                                        ("S", 5, 2, True, (0, 256)),
                                                        ("S", 5, 2, True, (1, 128)),
-                                                       ("S", 5, 2, True, (1, 512, 0))])
+                                                       ("S", 5, 2, True, (1, 512, 0)),
+                                                       ("S", 5, 2, True, (1, 512, 3)),
+                                                       ("W", 4, 2, True, (1, 512, 3))])
 def test_adjoint_matches_dual_numbers(gpu, cls, nsteps, ckpt, npb, sweep):
     """Reverse mode on the GPU against forward mode on the CPU oracle: the
     directional derivatives along 8 random (u0, frct) directions agree to
     1e-9 and the loss to 1e-12; u is back at u0 afterwards.  sweep: the LU
     sweep schedule (hyperplane launches, or a workgroup per zone of the given
-    size, and optionally its forwarding level) that both the forward and the
-    transposed sweeps follow; None keeps the default (a workgroup per zone
-    handing hyperplanes on through LDS and loading a hyperplane ahead)."""
+    size, and optionally its forwarding level, 3 handing hyperplanes on
+    through LDS and loading a hyperplane ahead) that both the forward and the
+    transposed sweeps follow; None keeps the default (at classes S and W a
+    workgroup per zone reading global memory)."""
     N = gpu.native
     zones, p = gpu.zones_for("LU", cls)
     ctx = gpu.Context("LU", zones, p.dt, omega=1.2, npb_exchange=npb)

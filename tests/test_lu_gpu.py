@@ -101,9 +101,14 @@ def test_full_run_strict_is_bit_exact_class_S(gpu, npb):
     ctx.close()
 
 
-@pytest.mark.parametrize("cls", ["S", "W"])
-def test_fma_kernels_track_oracle_per_step(gpu, cls):
+@pytest.mark.parametrize("cls,fwd", [("S", None), ("W", None), ("S", 3), ("W", 3)])
+def test_fma_kernels_track_oracle_per_step(gpu, cls, fwd):
+    """The FMA kernels' norms within 1e-12 of the oracle's after each of ten
+    iterations, with the default schedule (at S and W a workgroup per zone
+    reading global memory) and with the padded-halo sweep (fwd = 3)."""
     ctx, zones, p = _ctx(gpu, cls, False)
+    if fwd is not None:
+        ctx.set_schedule(gpu.native.SCHED_LU_FORWARD, fwd)
     o = _oracle(p)
     ctx.init_exact()
     for s in range(10):
