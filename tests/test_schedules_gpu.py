@@ -1,0 +1,54 @@
+"""Schedules that change only which workgroup runs which task give the same
+bits: the persistent solve grid (a few workgroups looping over the line
+tasks, NPBMZ_SCHED_SOLVE_GRID) against one workgroup per task, for the BT and
+SP steps (one-sided and two-sided solves) and their adjoints.  No counterpart
+in the reference, which runs one Julia thread per zone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(gpu, bench, cls, strict, grid, twist, nsteps=4):
+    N = gpu.native
+    zones, p = gpu.zones_for(bench, cls)
+    ctx = gpu.Context(bench, zones, p.dt, strict=strict)
+    if twist is not None:
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, twist)
+    if grid is not None:
+        ctx.set_schedule(N.SCHED_SOLVE_GRID, grid)
+    ctx.init_exact()
+    ctx.step(nsteps, "batched")
+    u = ctx.get_all(0)
+    ctx.close()
+    return u
+
+
+@pytest.mark.parametrize("bench,cls,strict,twist", [("BT", "W", True, None), ("BT", "W", False, 7),
+                                                    ("SP", "W", True, None), ("SP", "W", False, 7),
+                                                    ("SP", "A", False, 0)])
+def test_persistent_solve_grid_gives_the_same_bits(gpu, bench, cls, strict, twist):
+    print("grid", bench, cls, strict, twist, flush=True)
+    ref = _run(gpu, bench, cls, strict, None, twist)
+    for grid in (7, 64):
+        got = _run(gpu, bench, cls, strict, grid, twist)
+        assert np.array_equal(got, ref), (grid, np.abs(got - ref).max())
+
+
+@pytest.mark.parametrize("bench", ["BT", "SP"])
+def test_persistent_solve_grid_adjoint_gives_the_same_bits(gpu, bench):
+    N = gpu.native
+    zones, p = gpu.zones_for(bench, "S")
+    res = []
+    for grid in (None, 5):
+        ctx = gpu.Context(bench, zones, p.dt)
+        if grid is not None:
+            ctx.set_schedule(N.SCHED_SOLVE_GRID, grid)
+        ctx.init_exact()
+        w = np.random.Generator(np.random.Philox(7)).random(ctx.size)
+        r = ctx.adjoint(6, 3, w)
+        res.append(r)
+        ctx.close()
+    assert res[0]["loss"] == res[1]["loss"]
+    assert np.array_equal(res[0]["grad_u0"], res[1]["grad_u0"])
+    assert np.array_equal(res[0]["grad_forcing"], res[1]["grad_forcing"])
