@@ -52,3 +52,26 @@ def test_persistent_solve_grid_adjoint_gives_the_same_bits(gpu, bench):
     assert res[0]["loss"] == res[1]["loss"]
     assert np.array_equal(res[0]["grad_u0"], res[1]["grad_u0"])
     assert np.array_equal(res[0]["grad_forcing"], res[1]["grad_forcing"])
+
+
+@pytest.mark.parametrize("bench", ["BT", "SP"])
+def test_tune_keeps_the_fields_and_the_tuned_run_verifies(gpu, bench):
+    """npbmz_tune times graph replays of candidate schedules on the live
+    context: the fields are as before afterwards, and a whole class W run on
+    the schedule it picked verifies against the class table."""
+    zones, p = gpu.zones_for(bench, "W")
+    ctx = gpu.Context(bench, zones, p.dt)
+    ctx.init_exact()
+    ctx.step(2, "batched")
+    before = [ctx.get_all(f) for f in (0, 1)]
+    ctx.tune(200)
+    after = [ctx.get_all(f) for f in (0, 1)]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    sched = ctx.describe()
+    ctx.init_exact()
+    ctx.step(p.niter, "graph")
+    xcr, xce, _ = ctx.norms()
+    ok, rel = gpu.verify.verify(bench, "W", p.dt, p.niter, xcr, xce)
+    assert ok, (sched, rel)
+    ctx.close()
