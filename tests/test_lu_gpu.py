@@ -152,6 +152,26 @@ def test_reference_exchange_run_matches_reference_as_written(gpu):
     assert abs(r["xci"] / oxci - 1) < 1e-10
 
 
+@pytest.mark.parametrize("npb", [False, True])
+def test_class_A_default_schedule_tracks_oracle_per_iteration(gpu, npb):
+    """Class A's default schedule (FMA arithmetic, the padded-halo zone
+    sweeps) with either exchange's planes: the norms within 1e-12 of the
+    oracle's after each of five iterations."""
+    ctx, zones, p = _ctx(gpu, "A", False, npb)
+    assert "lds-halo" in ctx.describe()
+    o = OracleLU.for_class(p, flags=OracleLU.NPB_EXCH if npb else 0, threads=16)
+    ctx.init_exact()
+    for s in range(5):
+        ctx.step(1, "batched")
+        o.step(1)
+        gxcr, gxce, gxci = ctx.norms()
+        oxcr, oxce, oxci = o.norms()
+        assert np.abs(gxcr / oxcr - 1).max() < 1e-12, s
+        assert np.abs(gxce / oxce - 1).max() < 1e-12, s
+        assert abs(gxci / oxci - 1) < 1e-12, s
+    ctx.close()
+
+
 @pytest.mark.parametrize("strict", [True, False])
 def test_schedules_and_modes_agree_class_A(gpu, strict):
     """Every schedule computes each point with the same operations in the
