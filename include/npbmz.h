@@ -60,7 +60,7 @@ extern "C" {
 /* step modes */
 #define NPBMZ_MODE_PERZONE 0 /* one launch per zone per phase */
 #define NPBMZ_MODE_BATCHED 1 /* all zones in one launch per phase */
-#define NPBMZ_MODE_GRAPH 2   /* batched, whole timestep replayed from a hipGraph */
+#define NPBMZ_MODE_GRAPH 2   /* batched, replayed from hipGraphs of 8 timesteps and of 1 */
 
 /* phases for npbmz_run_phase (testing and profiling) */
 #define NPBMZ_PHASE_EXCH 0  /* exch_qbc face copy */

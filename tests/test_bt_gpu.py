@@ -93,13 +93,15 @@ def test_full_run_strict_is_bit_exact_class_S(gpu, pair, twist):
 
 
 def test_modes_agree_bitwise(gpu):
-    """Per-zone launches, batched launches and the hipGraph replay run the
-    same kernels on the same data: identical results."""
+    """Per-zone launches, batched launches and the hipGraph replays run the
+    same kernels on the same data: identical results.  19 steps in two calls
+    replay the eight-step graph twice and the one-step graph three times."""
     res = []
     for mode in ("perzone", "batched", "graph"):
         ctx, zones, p = _ctx(gpu, "W", False)
         ctx.init_exact()
         ctx.step(10, mode)
+        ctx.step(9, mode)
         res.append([ctx.get_dense(z, 0) for z in range(zones.count)])
         ctx.close()
     for r in res[1:]:
