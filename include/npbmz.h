@@ -121,7 +121,10 @@ int npbmz_init_exact(npbmz_ctx* ctx);
  * ssor (LU).  Replaces the STEP loop body of perform
  * (src/BT/bt-node.jl:280-417, src/SP/sp-node.jl:252-391,
  * src/LU/lu-node.jl:338-428).  If elapsed_ms is not NULL it receives the
- * HIP-event time of the whole call (inputs already resident in HBM). */
+ * HIP-event time of the whole call (inputs already resident in HBM).  In
+ * NPBMZ_MODE_GRAPH the first call captures a one-step graph, and the first
+ * call with nsteps >= 8 an eight-step graph, before the timed region; a
+ * schedule change discards both. */
 int npbmz_step(npbmz_ctx* ctx, int32_t nsteps, int32_t mode, double* elapsed_ms);
 
 /* Run one phase over all zones (testing and profiling). */
