@@ -19,8 +19,8 @@ def go(bench="BT", cls="S", mode="graph", niter=None, strict=False, warmup=True,
     k = 2..nz-1 as NPB-MZ does; the reference's own planes k = 1..nz-2
     (the default) do not reproduce its verification tables.
     mode: "perzone" (one launch per zone per phase), "batched" (one launch per
-    phase over all zones) or "graph" (batched timestep replayed from a
-    hipGraph).  strict: use the kernels built without FMA contraction
+    phase over all zones) or "graph" (batched timesteps replayed from
+    hipGraphs of eight steps and of one).  strict: use the kernels built without FMA contraction
     (bit-identical arithmetic to the reference's CPU path).
     Returns a dict with the norms, verification verdict, time and Mop/s.
     """
