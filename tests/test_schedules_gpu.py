@@ -75,3 +75,25 @@ def test_tune_keeps_the_fields_and_the_tuned_run_verifies(gpu, bench):
     ok, rel = gpu.verify.verify(bench, "W", p.dt, p.niter, xcr, xce)
     assert ok, (sched, rel)
     ctx.close()
+
+
+@pytest.mark.parametrize("bench", ["BT", "SP"])
+def test_schedule_change_discards_both_step_graphs(gpu, bench):
+    """Graph mode replays a captured eight-step graph and a one-step graph;
+    a schedule change between calls must discard both.  The two-sided and
+    the one-wave solves round differently with FMA arithmetic, so a stale
+    graph would show as different bits from the same sequence launched
+    batched."""
+    N = gpu.native
+    zones, p = gpu.zones_for(bench, "W")
+    res = []
+    for mode in ("batched", "graph"):
+        ctx = gpu.Context(bench, zones, p.dt)
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, 7)
+        ctx.init_exact()
+        ctx.step(10, mode)
+        ctx.set_schedule(N.SCHED_SOLVE_TWIST, 0)
+        ctx.step(9, mode)
+        res.append(ctx.get_all(0))
+        ctx.close()
+    assert np.array_equal(res[0], res[1])
