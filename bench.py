@@ -35,12 +35,46 @@ def face_points(zones):
     return n
 
 
-def cpu_baseline(params, budget_s, threads):
-    """The CPU oracle (C++ restatement of the reference's Julia BT path) timed
-    on this host's cores over a bounded sample of the same workload."""
+def host_cpu():
+    """This host's CPU model, the CPUs this process may run on, and the thread
+    budget the environment grants (OMP_NUM_THREADS, if set)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return model, avail, (min(avail, cap) if cap > 0 else avail)
+
+
+def oracle_full_run(params, threads):
+    """The CPU oracle (C++ restatement of the reference's Julia BT path) over
+    the whole class run: one warm-up step, re-initialisation (as the
+    reference's node does, src/BT/bt-node.jl:138-253), then niter timed
+    steps and the node-side norms.  Returns (seconds, xcr, xce)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleBT
-    import npbapps_amd as P
+    o = OracleBT.for_class(params, threads=threads)
+    o.step(1)
+    o.close()
+    o = OracleBT.for_class(params, threads=threads)
+    t0 = time.perf_counter()
+    o.step(params.niter)
+    t = time.perf_counter() - t0
+    xcr, xce = o.norms()
+    o.close()
+    return t, xcr, xce
+
+
+def oracle_sample(params, budget_s, threads):
+    """The oracle timed over a bounded number of steps (about budget_s)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleBT
     o = OracleBT.for_class(params, threads=threads)
     t0 = time.perf_counter()
     o.step(1)
@@ -50,12 +84,28 @@ def cpu_baseline(params, budget_s, threads):
     o.step(n)
     t = time.perf_counter() - t0
     o.close()
+    return n, t
+
+
+def cpu_baseline(P, params, budget_s):
+    """The reported CPU baseline: the oracle's whole class B run (200 steps)
+    on this host's cores, OpenMP over zones as the reference threads over
+    zones, plus a bounded single-core sample.  Also returns the oracle's
+    end-of-run norms, which the GPU run is checked against."""
     zones, _ = P.zones_for("BT", "B")
     mflop = P.mflops_per_step("BT", zones)
-    return {"value": round(mflop * n / t, 2), "unit": "Mop/s", "cores": threads,
+    model, avail, threads = host_cpu()
+    t, xcr, xce = oracle_full_run(params, threads)
+    n1, t1 = oracle_sample(params, budget_s, 1)
+    base = {"value": round(mflop * params.niter / t, 2), "unit": "Mop/s", "cores": threads,
             "kind": "port",
-            "sample": f"BT-MZ class B, {n} timesteps after 1 warm-up step, "
-                      f"{threads} OpenMP threads over zones, {t:.2f} s"}
+            "sample": f"BT-MZ class B, the whole {params.niter}-step run after 1 warm-up step, "
+                      f"{threads} OpenMP threads over zones, {t:.2f} s",
+            "cpu_model": model, "cpus_available": avail,
+            "single_core": {"value": round(mflop * n1 / t1, 2), "unit": "Mop/s", "cores": 1,
+                            "sample": f"BT-MZ class B, {n1} timesteps after 1 warm-up step, "
+                                      f"1 thread, {t1:.2f} s"}}
+    return base, xcr, xce
 
 
 def adjoint_case(P):
@@ -202,10 +252,12 @@ def main():
     ctx.step(W, "graph")
     ctx.init_exact()
     t_graph = ctx.step(K, "graph")
-    xcr, xce, _ = ctx.norms()
-    verified = None
-    if K == p.niter:
-        verified, _ = P.verify.verify("BT", "B", p.dt, K, xcr, xce)
+    # the whole class run on the same context and schedule, outside the timed
+    # region: its norms are checked against the class table and the oracle
+    ctx.init_exact()
+    t_full = ctx.step(p.niter, "graph")
+    gxcr, gxce, _ = ctx.norms()
+    verified, rel = P.verify.verify("BT", "B", p.dt, p.niter, gxcr, gxce)
 
     t_perzone = None
     if not args.no_perzone:
@@ -232,10 +284,19 @@ def main():
     dom = max(PHASES[1:], key=lambda n: per_launch_s[n])
     alg_bytes = FIELDS_PER_PHASE[dom] * 40 * npts
     achieved = alg_bytes / per_launch_s[dom] / 1e9
-    traffic = None
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
+    # on the copy kernel), used only while the kernel sources hash as they did
+    # when it was measured
+    traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", "bt_class_b_traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(dom)
+        from npbapps_amd.rocprof_summary import kernel_sources_sha256
+        rec = json.load(open(tfile))
+        if rec.get("kernels_sha256") == kernel_sources_sha256():
+            traffic = rec.get(dom)
+            traffic_src = {"head": rec.get("head"), "kernel": rec.get("kernels", {}).get(dom)}
+        else:
+            traffic_src = {"head": rec.get("head"), "stale": True}
 
     extra = {}
     if not args.no_extra:
@@ -265,6 +326,7 @@ def main():
                    "mode": "batched + hipGraph", "fp": "fma", "schedule": sched},
         "grid_pt_steps_per_s": round(npts * K / t_graph, 1),
         "verified": verified,
+        "oracle_rel_max": None,
         "batched_mops": round(mflop_step * K / t_batched, 2),
         "perzone": None if t_perzone is None else {
             "value": round(mflop_step * K / t_perzone, 2), "unit": "Mop/s",
@@ -280,6 +342,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_measured_on": traffic_src,
                      "algorithmic_bytes": alg_bytes, "copy_kernel": copy,
                      # the same launch priced with its measured HBM bytes (workspace included)
                      "measured_GBps": None if traffic is None
@@ -288,12 +351,18 @@ def main():
                      else round(traffic / per_launch_s[dom] / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": None,
     }
+    out["full_run"] = {"steps": p.niter, "ms_per_step": round(t_full / p.niter * 1e3, 5),
+                       "verified": verified,
+                       "class_table_rel_max": max(max(rel["xcr"]), max(rel["xce"])),
+                       "oracle_rel_max": None}
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds, threads)
-        # the same restatement on one core, for the per-core figure
-        one = cpu_baseline(p, args.cpu_seconds, 1)
-        out["cpu_baseline"]["single_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
+        import numpy as np
+        base, oxcr, oxce = cpu_baseline(P, p, args.cpu_seconds)
+        out["cpu_baseline"] = base
+        orel = max(np.abs(np.asarray(gxcr) / oxcr - 1).max(),
+                   np.abs(np.asarray(gxce) / oxce - 1).max())
+        out["full_run"]["oracle_rel_max"] = float(orel)
+        out["oracle_rel_max"] = float(orel)
     print(json.dumps(out))
 
 

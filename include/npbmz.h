@@ -172,7 +172,9 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     points, else 0) */
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup
                                     (default 512) */
-#define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else grid size */
+#define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else the size
+                                    (1..65536) of a persistent grid striding
+                                    over the tasks; NPBMZ_EINVAL otherwise */
 #define NPBMZ_SCHED_SOLVE_PAIR 3 /* BT: bit d set: direction d solves each 64 lines
                                     with a builder and an eliminator wave (default 2) */
 #define NPBMZ_SCHED_SOLVE_TWIST 4 /* BT and SP, FMA arithmetic only (ignored with

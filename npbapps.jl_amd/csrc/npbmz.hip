@@ -132,6 +132,7 @@ constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
 constexpr int kGraphGroup = 8; // timesteps per replay of the grouped graph
+constexpr int kMaxSolveGrid = 65536; // largest persistent solve grid accepted
 
 // LU sweep schedules: one launch per hyperplane over all zones, or one
 // workgroup per zone walking all hyperplanes of the sweep.
@@ -178,6 +179,7 @@ struct npbmz_ctx {
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Face* faces = nullptr;
   double* partial = nullptr;
+  size_t partial_n = 0;     // blocks the partial sums are sized for
   double* zpart = nullptr;  // one value per zone (LU surface integrals)
   std::vector<double> zn_rhs, zn_err;
   hipStream_t stream = nullptr;
@@ -327,32 +329,47 @@ int lines_of(const npbmz_ctx* c, int z, int dir) {
 }
 
 // Build a task table: zones in `order`, each zone's `count(z)` items cut into
-// tasks of `per` items.
+// tasks of `per` items.  The new device table is allocated before the old
+// one is freed, and t changes only on success, so a failure leaves the
+// previous table in place and usable.
 template <class F>
 int build_table(npbmz_ctx* ctx, Table& t, const std::vector<int>& order, int per, F count) {
   std::vector<Task> h;
   int nid = 0;
   for (int z : order) nid = std::max(nid, z + 1);
-  t.zfirst.assign(nid, 0);
-  t.zcount.assign(nid, 0);
+  std::vector<int> zfirst(nid, 0), zcount(nid, 0);
   for (int z : order) {
     int n = count(z);
     int nt = (n + per - 1) / per;
-    t.zfirst[z] = (int)h.size();
-    t.zcount[z] = nt;
+    zfirst[z] = (int)h.size();
+    zcount[z] = nt;
     for (int q = 0; q < nt; ++q) h.push_back(Task{z, q * per});
   }
-  if (t.d) { (void)hipFree(t.d); t.d = nullptr; }
+  Task* d = nullptr;
+  if (!h.empty()) {
+    CK(hipMalloc(&d, sizeof(Task) * h.size()));
+    hipError_t e = hipMemcpy(d, h.data(), sizeof(Task) * h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(ctx, NPBMZ_EHIP, std::string("task table upload: ") + hipGetErrorString(e));
+    }
+  }
+  if (t.d) (void)hipFree(t.d);
+  t.d = d;
   t.n = (int)h.size();
-  if (t.n == 0) return NPBMZ_OK;
-  CK(hipMalloc(&t.d, sizeof(Task) * h.size()));
-  CK(hipMemcpy(t.d, h.data(), sizeof(Task) * h.size(), hipMemcpyHostToDevice));
+  t.zfirst.swap(zfirst);
+  t.zcount.swap(zcount);
   return NPBMZ_OK;
 }
 
+// (Re)build every task table and the buffers sized by them.  Captured graphs
+// hold the old tables' addresses, so they are dropped first, whatever the
+// outcome.  Each buffer is replaced only once its successor is allocated: a
+// failure (EHIP/ENOMEM) leaves a consistent, usable context.
 int build_tables(npbmz_ctx* ctx) {
   int rc;
   auto& c = *ctx;
+  drop_graphs(c);
   if ((rc = build_table(ctx, c.t_int, c.order, kPtNT, [&](int z) {
          return (c.nx[z] - 2) * (c.ny[z] - 2) * (c.nz[z] - 2);
        })))
@@ -381,9 +398,9 @@ int build_tables(npbmz_ctx* ctx) {
       mr = std::max(mr, nl - 2);
     }
     c.maxrows[d] = mr;
-    int slots = c.t_line[d].n;
-    for (int z = 0; z < c.nz_; ++z) slots = std::max(slots, c.t_line[d].zcount[z]);
-    if (c.solve_grid[d] > 0) slots = std::max(slots, c.solve_grid[d]);
+    // one scratch slice per workgroup: a launch never has more workgroups
+    // than tasks (the persistent grid is min(tasks, solve_grid))
+    const int slots = c.t_line[d].n;
     // reduced coefficients kept per row for the back sweep: BT's 5x5 C'
     // block and reduced rhs (fast::bt::kRowPairs double pairs), SP's
     // lhs(4:5) of three systems
@@ -391,9 +408,13 @@ int build_tables(npbmz_ctx* ctx) {
     slot = std::max(slot, (size_t)slots * mr * per_row * 64);
   }
   if (slot > c.cscr_slot) {
+    double* n = nullptr;
+    hipError_t e = hipMalloc(&n, slot * sizeof(double));
+    if (e != hipSuccess)
+      return fail(ctx, e == hipErrorOutOfMemory ? NPBMZ_ENOMEM : NPBMZ_EHIP,
+                  std::string("line scratch: ") + hipGetErrorString(e));
     if (c.cscr) (void)hipFree(c.cscr);
-    c.cscr = nullptr;
-    CK(hipMalloc(&c.cscr, slot * sizeof(double)));
+    c.cscr = n;
     c.cscr_slot = slot;
   }
   // Faces.  BT and SP copy the interior of each face, j (or i) in 1..n-2 on
@@ -439,12 +460,17 @@ int build_tables(npbmz_ctx* ctx) {
     return rc;
   if ((rc = build_table(ctx, c.t_face2, fb, kExchNT, [&](int f) { return fh[f].npts; })))
     return rc;
-  int pmax = std::max(c.t_int.n, c.t_all.n);
-  if (c.partial) (void)hipFree(c.partial);
-  c.partial = nullptr;
-  CK(hipMalloc(&c.partial, sizeof(double) * 10 * (size_t)std::max(pmax, 1)));
+  // the norms' per-block partial sums; the point tables' sizes do not depend
+  // on the order, so this is allocated once
+  const size_t pmax = (size_t)std::max(std::max(c.t_int.n, c.t_all.n), 1);
+  if (pmax > c.partial_n) {
+    double* n = nullptr;
+    CK(hipMalloc(&n, sizeof(double) * 10 * pmax));
+    if (c.partial) (void)hipFree(c.partial);
+    c.partial = n;
+    c.partial_n = pmax;
+  }
   if (!c.zpart) CK(hipMalloc(&c.zpart, sizeof(double) * (size_t)c.nz_));
-  drop_graphs(c);
   return NPBMZ_OK;
 }
 
@@ -1690,13 +1716,13 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
         return fail(ctx, NPBMZ_EINVAL, "BT/SP two-sided solve mask is 0..7");
       c.solve_twist = value;
       break;
-    case NPBMZ_SCHED_SOLVE_GRID: {
-      if (c.bench == NPBMZ_LU || value < 0) return fail(ctx, NPBMZ_EINVAL, "bad solve grid");
+    case NPBMZ_SCHED_SOLVE_GRID:
+      // a launch takes min(tasks, grid) workgroups, so the line scratch
+      // (one slice per task) never depends on it
+      if (c.bench == NPBMZ_LU || value < 0 || value > kMaxSolveGrid)
+        return fail(ctx, NPBMZ_EINVAL, "solve grid must be 0..65536 (BT/SP)");
       for (int d = 0; d < 3; ++d) c.solve_grid[d] = value;
-      int rc = build_tables(ctx);
-      if (rc) return rc;
       break;
-    }
     default: return fail(ctx, NPBMZ_EINVAL, "unknown schedule knob");
   }
   drop_graphs(c);

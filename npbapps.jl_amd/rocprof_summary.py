@@ -1,0 +1,130 @@
+"""Summaries of rocprofv3 output for libnpbmz's kernels.
+
+rocprofv3 writes CSV files under its -d directory (per host and process):
+``*_kernel_stats.csv`` for ``--kernel-trace --stats`` and
+``*_counter_collection.csv`` for ``--pmc``.  This module turns them into the
+per-kernel tables kept under profiles/ and into the HBM-traffic record that
+bench.py reads for ``roofline.traffic``.
+
+The traffic record carries the commit and a hash of the kernel sources it was
+measured on; bench.py uses it only while the sources still hash the same, so
+a kernel change can never be priced with another kernel's traffic.
+
+Usage (after two --pmc runs, FETCH_SIZE then WRITE_SIZE, of the same command):
+    python -m npbapps_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR OUT.json
+    python -m npbapps_amd.rocprof_summary stats TRACE_DIR
+"""
+import csv
+import glob
+import hashlib
+import json
+import os
+import subprocess
+import sys
+from collections import defaultdict
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+# the sources whose text determines the BT kernels' memory traffic
+KERNEL_SOURCES = ("csrc/npbmz.hip", "csrc/bt_kernels.inc", "csrc/npbmz_types.h")
+COPY_KERNEL = "copy_probe_kernel"
+# phase name (bench.py) -> kernel name prefix of its default-schedule kernel
+PHASE_KERNELS = {
+    "exch": "npbmz::exch_kernel",
+    "rhs": "npbmz::fast::bt::rhs_kernel",
+    "x_solve": "<0, false>",
+    "y_solve": "<1, false>",
+    "z_solve+add": "<2, true>",
+}
+
+
+def kernel_sources_sha256():
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(HERE, rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def git_head():
+    try:
+        return subprocess.run(["git", "-C", ROOT, "rev-parse", "--short=12", "HEAD"],
+                              capture_output=True, text=True, check=True).stdout.strip()
+    except (OSError, subprocess.CalledProcessError):
+        return None
+
+
+def _rows(d, suffix):
+    files = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    for fn in files:
+        with open(fn, newline="") as f:
+            yield from csv.DictReader(f)
+
+
+def counter_per_dispatch(d, counter):
+    """Average value of `counter` per dispatch, by kernel name."""
+    tot, cnt = defaultdict(float), defaultdict(set)
+    for r in _rows(d, "counter_collection.csv"):
+        if r.get("Counter_Name") != counter:
+            continue
+        k = r["Kernel_Name"]
+        tot[k] += float(r["Counter_Value"])
+        cnt[k].add(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+    return {k: tot[k] / max(1, len(cnt[k])) for k in tot}
+
+
+def kernel_stats(d):
+    """(name, calls, avg us, min us, max us, percent) rows, slowest total first."""
+    out = []
+    for r in _rows(d, "kernel_stats.csv"):
+        out.append((r["Name"], int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                    float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    return sorted(out, key=lambda x: -x[1] * x[2])
+
+
+def traffic(fetch_dir, write_dir, copy_bytes):
+    """Scaled HBM bytes per dispatch of each BT phase kernel.  FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) are each scaled by what they read for the
+    copy kernel, whose reads and writes are copy_bytes each."""
+    fe = counter_per_dispatch(fetch_dir, "FETCH_SIZE")
+    wr = counter_per_dispatch(write_dir, "WRITE_SIZE")
+    ck = [k for k in fe if COPY_KERNEL in k]
+    if not ck:
+        raise ValueError("the copy kernel is not in the FETCH_SIZE run")
+    fscale = copy_bytes / (fe[ck[0]] * 1024.0)
+    wscale = copy_bytes / (wr[ck[0]] * 1024.0)
+    rec = {"head": git_head(), "kernels_sha256": kernel_sources_sha256(),
+           "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "kernels": {}}
+    for phase, needle in PHASE_KERNELS.items():
+        kf = [k for k in fe if needle in k and ("exch" in k or "bt::" in k)]
+        if not kf:
+            continue
+        k = kf[0]
+        b = fe[k] * 1024.0 * fscale + wr.get(k, 0.0) * 1024.0 * wscale
+        rec[phase] = int(round(b))
+        rec["kernels"][phase] = k
+    return rec
+
+
+def main(argv):
+    if len(argv) >= 4 and argv[0] == "traffic":
+        copy = int(argv[4]) if len(argv) > 4 else 512 << 20
+        rec = traffic(argv[1], argv[2], copy)
+        with open(argv[3], "w") as f:
+            json.dump(rec, f, indent=1)
+            f.write("\n")
+        print(json.dumps(rec, indent=1))
+        return 0
+    if len(argv) == 2 and argv[0] == "stats":
+        print("| kernel | calls | avg us | min us | max us | % |\n|---|---|---|---|---|---|")
+        for n, c, a, lo, hi, pc in kernel_stats(argv[1]):
+            print(f"| `{n}` | {c} | {a:.1f} | {lo:.1f} | {hi:.1f} | {pc:.1f} |")
+        return 0
+    print(__doc__)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
