@@ -145,6 +145,23 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]);
  * surface integral (src/LU/lu-node.jl:430-464). */
 int npbmz_norms(npbmz_ctx* ctx, double xcr[5], double xce[5], double* xci);
 
+/* npbmz_norms on caller-owned device fields, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  u, forcing and rhs are device arrays in the
+ * library's whole-run layout (every zone's nx*ny*nz*5 values, zone after
+ * zone, each zone [k][j][i][m] with m fastest).  BT/SP: rhs receives
+ * compute_rhs of u without an exchange, as npbmz_norms leaves it in the
+ * context's rhs; LU: rhs holds rsd (read only) and forcing is not used.  The
+ * results equal npbmz_norms' on the same state bit for bit.  The kernels'
+ * partial sums live in a static 2 MiB device array, one per GPU, shared by
+ * every call on that GPU: calls that overlap on one GPU race, and problems
+ * whose partial sums exceed it (about 23,000 blocks of 128 points for BT/SP,
+ * e.g. BT-MZ class C) fail with NPBMZ_EINVAL.  npbmz_zone_norms does not see
+ * these calls.  No hipMalloc, hipFree or other allocation. */
+int npbmz_norms_noalloc(npbmz_ctx* ctx, const double* u, const double* forcing, double* rhs,
+                        double xcr[5], double xce[5], double* xci);
+
 /* Per-zone raw residual/error norms of the last npbmz_norms call
  * (nzones x 5 each, zone-major); either pointer may be NULL. */
 int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err);

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -120,6 +121,60 @@ __device__ __forceinline__ double recip(double x) {
 #include "lu_adjoint.inc"
 }  // namespace fast
 
+// --------------------------------------------- the allocation-free norms
+// Scratch of npbmz_norms_noalloc: the norm kernels' per-block partial sums,
+// each zone's block ranges, per-zone norms and the final sums.  One copy per
+// GPU, held for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchDoubles = (size_t(2) << 20) / sizeof(double);  // 2 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Final reductions of npbmz_norms on the device, in the host path's order:
+// per zone, the blocks' partial sums in block order from 0.0; the zone's
+// rms; then the sums over zones in zone order.  BT/SP (src/BT/verify.jl:
+// 434-499): pe, pr are error_norm's (all points) and rhs_norm's (interior)
+// partials, 5 per block; xcr = sum of rms_rhs/dt, xce = sum of rms_err.  LU
+// (src/LU/lu-node.jl:433-458): pe holds 10 per block (ERROR, then l2norm of
+// rsd), zp each zone's surface integral; xcr and xce are plain sums.
+// rng: first block and block count per zone of pe's table, then of pr's.
+// out: xcr[5], xce[5], xci.  One workgroup; zn holds nz x 10 values.
+__global__ __launch_bounds__(256) void norms_reduce_kernel(
+    const double* __restrict__ pe, const double* __restrict__ pr, const double* __restrict__ zp,
+    const int32_t* __restrict__ rng, const ZoneDev* __restrict__ zones, int nz, int lu,
+    double dt, double* __restrict__ zn, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+    const ZoneDev Z = zones[z];
+    const double vol = double(Z.nx - 2) * double(Z.ny - 2) * double(Z.nz - 2);
+    double se[10];
+    for (int q = 0; q < 10; ++q) se[q] = 0.0;
+    const int fa = rng[z], ca = rng[nz + z];
+    for (int b = 0; b < ca; ++b)
+      for (int q = 0; q < (lu ? 10 : 5); ++q) se[q] += pe[(size_t)(fa + b) * (lu ? 10 : 5) + q];
+    if (!lu) {
+      const int fi = rng[2 * nz + z], ci = rng[3 * nz + z];
+      for (int b = 0; b < ci; ++b)
+        for (int m = 0; m < 5; ++m) se[5 + m] += pr[(size_t)(fi + b) * 5 + m];
+    }
+    for (int q = 0; q < 10; ++q) zn[(size_t)z * 10 + q] = sqrt(se[q] / vol);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double xcr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, xce[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, frc = 0.0;
+  for (int z = 0; z < nz; ++z) {
+    for (int m = 0; m < 5; ++m) {
+      xcr[m] = lu ? xcr[m] + zn[(size_t)z * 10 + 5 + m] : xcr[m] + zn[(size_t)z * 10 + 5 + m] / dt;
+      xce[m] = xce[m] + zn[(size_t)z * 10 + m];
+    }
+    if (lu) frc = frc + zp[z];
+  }
+  for (int m = 0; m < 5; ++m) {
+    out[m] = xcr[m];
+    out[5 + m] = xce[m];
+  }
+  out[10] = frc;
+}
+
 }  // namespace npbmz
 
 using namespace npbmz;
@@ -190,6 +245,11 @@ struct npbmz_ctx {
   hipGraph_t graph_n = nullptr;
   std::string err;
   std::string sched;
+  // npbmz_norms_noalloc: this device's copy of the static scratch, and the
+  // per-zone block ranges it uploads there (built with the task tables)
+  double* scratch = nullptr;
+  std::string scratch_err;
+  std::vector<int32_t> noalloc_rng;
 };
 
 namespace {
@@ -471,6 +531,16 @@ int build_tables(npbmz_ctx* ctx) {
     c.partial_n = pmax;
   }
   if (!c.zpart) CK(hipMalloc(&c.zpart, sizeof(double) * (size_t)c.nz_));
+  // npbmz_norms_noalloc's block ranges: first block and count per zone of
+  // the error norm's table, then of the residual norm's (LU: one table)
+  const Table& ta = c.bench == NPBMZ_LU ? c.t_int : c.t_all;
+  c.noalloc_rng.assign((size_t)4 * c.nz_, 0);
+  for (int z = 0; z < c.nz_; ++z) {
+    c.noalloc_rng[z] = ta.zfirst[z];
+    c.noalloc_rng[c.nz_ + z] = ta.zcount[z];
+    c.noalloc_rng[2 * c.nz_ + z] = c.t_int.zfirst[z];
+    c.noalloc_rng[3 * c.nz_ + z] = c.t_int.zcount[z];
+  }
   return NPBMZ_OK;
 }
 
@@ -482,14 +552,19 @@ bool strict(const npbmz_ctx* c) { return (c->flags & NPBMZ_FLAG_STRICT_FP) != 0;
 
 // compute_rhs.  For SP the timestep's rhs phase is compute_rhs + txinvr;
 // plain = true gives compute_rhs alone (SP's verify, src/SP/verify.jl:436).
-int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false, double* r0out = nullptr) {
+// u, frc, rhs default to the context's fields.
+int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false, double* r0out = nullptr,
+          const double* u = nullptr, const double* frc = nullptr, double* rhs = nullptr) {
   auto& c = *ctx;
   const Table& t = c.t_int;
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
   if (n <= 0) return NPBMZ_OK;
   const bool tx = c.bench == NPBMZ_SP && !plain;
+  if (!u) u = c.u;
+  if (!frc) frc = c.frc;
+  if (!rhs) rhs = c.rhs;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, c.u, c.frc, c.rhs, c.zd,
+    hipLaunchKernelGGL(kern, dim3(n), dim3(kPtNT), 0, c.stream, u, frc, rhs, c.zd,
                        t.d + first, c.K, tx ? r0out : nullptr);
   };
   if (strict(ctx)) {
@@ -1069,6 +1144,32 @@ int ensure_graph(npbmz_ctx* ctx, int nsteps) {
   return NPBMZ_OK;
 }
 
+// This device's address of the static scratch, looked up once per device
+// (with that device current, as in npbmz_create) and kept for every later
+// context on it.  A failed lookup is kept as a message: only
+// npbmz_norms_noalloc needs the scratch, and it reports the failure.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+void bind_scratch(npbmz_ctx& c, int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    c.scratch_err = "device ordinal beyond the scratch table";
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      c.scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  c.scratch = g_scratch_dev[dev];
+}
+
 void describe(npbmz_ctx* c) {
   char b[256];
   if (c->bench == NPBMZ_LU)
@@ -1159,6 +1260,7 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
 
   npbmz_ctx* ctx = new npbmz_ctx();
   auto& c = *ctx;
+  bind_scratch(c, dev);
   c.bench = p->bench; c.flags = flags; c.nz_ = n;
   c.x_zones = p->x_zones; c.y_zones = p->y_zones;
   c.gx = p->gx_size; c.gy = p->gy_size; c.gz = p->gz_size;
@@ -1591,6 +1693,71 @@ int npbmz_zone_norms(npbmz_ctx* ctx, double* rms_rhs, double* rms_err) {
   if (ctx->zn_rhs.empty()) return fail(ctx, NPBMZ_EINVAL, "npbmz_norms has not run");
   if (rms_rhs) std::memcpy(rms_rhs, ctx->zn_rhs.data(), ctx->zn_rhs.size() * sizeof(double));
   if (rms_err) std::memcpy(rms_err, ctx->zn_err.data(), ctx->zn_err.size() * sizeof(double));
+  return NPBMZ_OK;
+}
+
+int npbmz_norms_noalloc(npbmz_ctx* ctx, const double* u, const double* forcing, double* rhs,
+                        double xcr[5], double xce[5], double* xci) {
+  if (!ctx) return NPBMZ_EINVAL;
+  auto& c = *ctx;
+  const bool lu = c.bench == NPBMZ_LU;
+  if (!u || !rhs || (!lu && !forcing) || !xcr || !xce)
+    return fail(ctx, NPBMZ_EINVAL, "null argument");
+  if (!c.scratch) return fail(ctx, NPBMZ_EHIP, "no scratch on this device: " + c.scratch_err);
+  const int nzn = c.nz_;
+  const Table& ta = lu ? c.t_int : c.t_all;
+  // scratch layout (doubles): partials A (LU 10 per block, else 5), partials B
+  // (BT/SP, 5 per interior block), per-zone pintgr (LU), per-zone norms
+  // (10 each), results (11), block ranges (4 int32 per zone)
+  const size_t na = (size_t)ta.n * (lu ? 10 : 5), nb = lu ? 0 : (size_t)c.t_int.n * 5;
+  const size_t nzp = lu ? (size_t)nzn : 0, nzn10 = (size_t)nzn * 10, nrng = ((size_t)4 * nzn + 1) / 2;
+  const size_t need = na + nb + nzp + nzn10 + 11 + nrng;
+  if (need > kScratchDoubles)
+    return fail(ctx, NPBMZ_EINVAL,
+                "problem too large for the no-allocation scratch (" + std::to_string(need) +
+                    " doubles needed, " + std::to_string(kScratchDoubles) + " available)");
+  double* pa = c.scratch;
+  double* pb = pa + na;
+  double* zp = pb + nb;
+  double* zn = zp + nzp;
+  double* res = zn + nzn10;
+  int32_t* rng = reinterpret_cast<int32_t*>(res + 11);
+  CK(hipMemcpyAsync(rng, c.noalloc_rng.data(), c.noalloc_rng.size() * sizeof(int32_t),
+                    hipMemcpyHostToDevice, c.stream));
+  if (lu) {
+    hipLaunchKernelGGL(exact::lu::norm_kernel<kPtNT>, dim3(ta.n), dim3(kPtNT), 0, c.stream, u,
+                       rhs, c.zd, ta.d, pa);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(exact::lu::pintgr_kernel<256>, dim3(nzn), dim3(256), 0, c.stream, u, c.zd,
+                       c.lk, zp);
+    CK(hipGetLastError());
+  } else {
+    // error_norm on u, compute_rhs of u without an exchange into rhs, then
+    // rhs_norm: the launches of npbmz_norms, on the caller's fields
+    if (strict(ctx))
+      hipLaunchKernelGGL(exact::bt::norm_kernel<kPtNT>, dim3(ta.n), dim3(kPtNT), 0, c.stream, u,
+                         rhs, c.zd, ta.d, pa, 0, c.K);
+    else
+      hipLaunchKernelGGL(fast::bt::norm_kernel<kPtNT>, dim3(ta.n), dim3(kPtNT), 0, c.stream, u,
+                         rhs, c.zd, ta.d, pa, 0, c.K);
+    CK(hipGetLastError());
+    int rc = k_rhs(ctx, -1, true, nullptr, u, forcing, rhs);
+    if (rc) return rc;
+    hipLaunchKernelGGL(exact::bt::norm_kernel<kPtNT>, dim3(c.t_int.n), dim3(kPtNT), 0, c.stream,
+                       u, rhs, c.zd, c.t_int.d, pb, 1, c.K);
+    CK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(norms_reduce_kernel, dim3(1), dim3(256), 0, c.stream, pa, pb, zp, rng, c.zd,
+                     nzn, lu ? 1 : 0, c.dt, zn, res);
+  CK(hipGetLastError());
+  double out[11];
+  CK(hipMemcpyAsync(out, res, sizeof out, hipMemcpyDeviceToHost, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  for (int m = 0; m < 5; ++m) {
+    xcr[m] = out[m];
+    xce[m] = out[5 + m];
+  }
+  if (xci) *xci = out[10];
   return NPBMZ_OK;
 }
 

@@ -33,6 +33,7 @@ EXPORTS = (
     "npbmz_get_field", "npbmz_init_exact", "npbmz_step", "npbmz_run_phase", "npbmz_phase_times",
     "npbmz_norms", "npbmz_zone_norms", "npbmz_tune", "npbmz_describe", "npbmz_size",
     "npbmz_probe_copy", "npbmz_set_schedule", "npbmz_adjoint", "npbmz_loss",
+    "npbmz_norms_noalloc",
 )
 
 
@@ -88,6 +89,7 @@ def load(path=LIB_PATH):
     lib.npbmz_set_schedule.argtypes = [P, i32, i32]
     lib.npbmz_adjoint.argtypes = [P, i32, i32, dp, dp, dp, dp, dp, dp]
     lib.npbmz_loss.argtypes = [P, dp, dp]
+    lib.npbmz_norms_noalloc.argtypes = [P, P, P, P, dp, dp, dp]
     for f in EXPORTS:
         if f not in ("npbmz_destroy", "npbmz_last_error"):
             getattr(lib, f).restype = ctypes.c_int
@@ -240,6 +242,17 @@ class Context:
         xci = np.zeros(1)
         self._check(self._lib.npbmz_norms(self._h, _dptr(xcr), _dptr(xce), _dptr(xci)),
                     "npbmz_norms")
+        return xcr, xce, float(xci[0])
+
+    def norms_noalloc(self, u_ptr, forcing_ptr, rhs_ptr):
+        """npbmz_norms on caller-owned device fields (device addresses, the
+        whole-run layout), with no allocation in the library."""
+        xcr = np.zeros(5)
+        xce = np.zeros(5)
+        xci = np.zeros(1)
+        self._check(self._lib.npbmz_norms_noalloc(self._h, u_ptr, forcing_ptr, rhs_ptr,
+                                                  _dptr(xcr), _dptr(xce), _dptr(xci)),
+                    "npbmz_norms_noalloc")
         return xcr, xce, float(xci[0])
 
     def zone_norms(self):
