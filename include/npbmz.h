@@ -216,6 +216,15 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     loads are issued a hyperplane ahead;
                                     otherwise as 2.  Default: 3 from 128 (j, k)
                                     pairs per hyperplane up, else 0 */
+#define NPBMZ_SCHED_SOLVE_ROWS 6 /* BT, FMA arithmetic only (ignored with
+                                    NPBMZ_FLAG_STRICT_FP): bit d set: direction d
+                                    solves 12 lines per wave with each line's five
+                                    block rows on five lanes, two-sided, keeping
+                                    the last reduced rows of each half in
+                                    registers (overrides TWIST and PAIR).
+                                    Default: x and y, and z while z's tasks
+                                    of 12 lines all fit one round of two waves
+                                    per SIMD */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

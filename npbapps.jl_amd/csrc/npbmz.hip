@@ -114,6 +114,7 @@ __device__ __forceinline__ double recip(double x) {
   return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
 }
 #include "bt_kernels.inc"
+#include "bt_rows.inc"
 #include "bt_adjoint.inc"
 #include "sp_kernels.inc"
 #include "sp_adjoint.inc"
@@ -185,6 +186,7 @@ namespace {
 constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
+constexpr int kRowR = 8;     // reduced rows per half line kept in registers (rows kernel)
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
 constexpr int kGraphGroup = 8; // timesteps per replay of the grouped graph
 constexpr int kMaxSolveGrid = 65536; // largest persistent solve grid accepted
@@ -225,6 +227,9 @@ struct npbmz_ctx {
   int solve_pair = 2;             // BT: bit d set: direction d uses solve_pair_kernel
   int solve_twist = 7;            // BT/SP, FMA only: bit d set: solve_twist_kernel (wins over pair)
   bool sp_twist_ok[3] = {false, false, false};  // SP: every line of direction d has >= 4 unknowns
+  int solve_rows = 0;             // BT, FMA only: bit d set: solve_rows_kernel (wins over twist);
+                                  // set at creation by size
+  int halfmax[3] = {0, 0, 0};     // rows of the longer half of the longest line per direction
   bool largest_first = true;      // task order: largest zones / longest lines first
   int lu_sweep = kSweepZone;      // LU sweep schedule
   int lu_nt = 256;                // threads per workgroup of the per-zone sweep
@@ -232,6 +237,7 @@ struct npbmz_ctx {
                                   // through LDS; 3: the padded-halo sweep where it applies
   size_t max_lds = 65536;         // dynamic LDS a zone-sweep workgroup may take
   Table t_int, t_all, t_line[3], t_face, t_face2;
+  Table t_row[3];  // BT: tasks of fast::bt::kRowLines lines for solve_rows_kernel
   Face* faces = nullptr;
   double* partial = nullptr;
   size_t partial_n = 0;     // blocks the partial sums are sized for
@@ -466,6 +472,17 @@ int build_tables(npbmz_ctx* ctx) {
     // lhs(4:5) of three systems
     const int per_row = c.bench == NPBMZ_SP ? 6 : 2 * fast::bt::kRowPairs;
     slot = std::max(slot, (size_t)slots * mr * per_row * 64);
+    if (c.bench == NPBMZ_BT) {
+      // the rows kernel: its own tasks (same zone and line order), and its
+      // spill of reduced rows beyond the last kRowR of each half, 6 doubles
+      // per lane and row for both waves of a task
+      if ((rc = build_table(ctx, c.t_row[d], ord, fast::bt::kRowLines,
+                            [&](int z) { return lines_of(ctx, z, d); })))
+        return rc;
+      c.halfmax[d] = (mr + 1) / 2;
+      const size_t spill_rows = (size_t)std::max(0, c.halfmax[d] - kRowR);
+      slot = std::max(slot, (size_t)c.t_row[d].n * spill_rows * 2 * 6 * 64);
+    }
   }
   if (slot > c.cscr_slot) {
     double* n = nullptr;
@@ -611,6 +628,14 @@ int k_solve_t(npbmz_ctx* ctx, int zone, double* xraw = nullptr) {
     else if (((c.solve_twist >> DIR) & 1) && c.sp_twist_ok[DIR])
       go_sp(fast::sp::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
     else go_sp(fast::sp::solve_kernel<DIR, FUSE>, kLineNT);
+  } else if (!strict(ctx) && ((c.solve_rows >> DIR) & 1)) {
+    const Table& tr = c.t_row[DIR];
+    const int f0 = zone < 0 ? 0 : tr.zfirst[zone], nr = zone < 0 ? tr.n : tr.zcount[zone];
+    if (nr <= 0) return NPBMZ_OK;
+    int gr = nr;
+    if (zone < 0 && c.solve_grid[DIR] > 0) gr = std::min(nr, c.solve_grid[DIR]);
+    hipLaunchKernelGGL((fast::bt::solve_rows_kernel<DIR, FUSE, kRowR>), dim3(gr), dim3(128), 0,
+                       c.stream, c.u, c.rhs, c.cscr, c.zd, tr.d + f0, nr, c.halfmax[DIR], c.K);
   } else if (!strict(ctx) && ((c.solve_twist >> DIR) & 1)) {
     go(fast::bt::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
   } else if ((c.solve_pair >> DIR) & 1) {
@@ -1190,10 +1215,10 @@ void describe(npbmz_ctx* c) {
                                             (c->sp_twist_ok[2] ? 4 : 0))),
              strict(c) ? "strict" : "fma");
   else
-    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d fp=%s", c->nz_,
-             c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
+    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d rows=%d fp=%s",
+             c->nz_, c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
              c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? 0 : c->solve_twist,
-             strict(c) ? "strict" : "fma");
+             strict(c) ? 0 : c->solve_rows, strict(c) ? "strict" : "fma");
   c->sched = b;
 }
 
@@ -1365,6 +1390,17 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       if (ok) c.max_lds = (size_t)v;
     }
   }
+  if (c.bench == NPBMZ_BT) {
+    // default, as measured on MI355X (ms/step, two-sided solves against the
+    // rows kernel in x and y, and in x, y and z): class S 0.057 / 0.043 /
+    // 0.042, W 0.082 / 0.065 / 0.063, A 0.173 / 0.148 / 0.153, B 0.396 /
+    // 0.328 / 0.332, C 1.421 / 1.318 / 1.331.  The rows kernel in z pays
+    // only while all of z's line tasks run in one round of two waves per SIMD
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    c.solve_rows = 3 | (c.t_row[2].n <= 4 * cus ? 4 : 0);
+  }
   if (c.bench == NPBMZ_SP) {
     // default: two-sided solves in the directions with fewer line tasks
     // than the chip has CUs, as measured on MI355X (class A, at most 225
@@ -1388,8 +1424,8 @@ void npbmz_destroy(npbmz_ctx* ctx) {
   auto& c = *ctx;
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   drop_graphs(c);
-  Table* ts[] = {&c.t_int,     &c.t_all,  &c.t_line[0], &c.t_line[1],
-                 &c.t_line[2], &c.t_face, &c.t_face2};
+  Table* ts[] = {&c.t_int,    &c.t_all,    &c.t_line[0], &c.t_line[1], &c.t_line[2],
+                 &c.t_face,   &c.t_face2,  &c.t_row[0],  &c.t_row[1],  &c.t_row[2]};
   for (Table* t : ts)
     if (t->d) (void)hipFree(t->d);
   void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart};
@@ -1780,6 +1816,8 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
     if (!strict(ctx)) knobs.push_back({NPBMZ_SCHED_SOLVE_TWIST, {0, 1, 3, 7}});
+    if (c.bench == NPBMZ_BT && !strict(ctx))
+      knobs.push_back({NPBMZ_SCHED_SOLVE_ROWS, {0, 1, 3, 7}});
     knobs.push_back({NPBMZ_SCHED_SOLVE_GRID, {0, 1024}});
   }
   auto get = [&](int which) -> int {
@@ -1789,6 +1827,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
       case NPBMZ_SCHED_LU_FORWARD: return c.lu_fwd;
       case NPBMZ_SCHED_SOLVE_PAIR: return c.solve_pair;
       case NPBMZ_SCHED_SOLVE_TWIST: return c.solve_twist;
+      case NPBMZ_SCHED_SOLVE_ROWS: return c.solve_rows;
       case NPBMZ_SCHED_SOLVE_GRID: return c.solve_grid[0];
       default: return c.largest_first ? 1 : 0;
     }
@@ -1882,6 +1921,11 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench == NPBMZ_LU || value < 0 || value > 7)
         return fail(ctx, NPBMZ_EINVAL, "BT/SP two-sided solve mask is 0..7");
       c.solve_twist = value;
+      break;
+    case NPBMZ_SCHED_SOLVE_ROWS:
+      if (c.bench != NPBMZ_BT || value < 0 || value > 7)
+        return fail(ctx, NPBMZ_EINVAL, "BT rows-over-lanes solve mask is 0..7");
+      c.solve_rows = value;
       break;
     case NPBMZ_SCHED_SOLVE_GRID:
       // a launch takes min(tasks, grid) workgroups, so the line scratch

@@ -26,6 +26,7 @@ PHASE_EXCH, PHASE_RHS, PHASE_XSOLVE, PHASE_YSOLVE, PHASE_ZSOLVE, PHASE_ADD = ran
 PHASE_SSOR_LOWER, PHASE_SSOR_UPPER, PHASE_SSOR_UPDATE = 2, 3, 4
 SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID, SCHED_SOLVE_PAIR, SCHED_SOLVE_TWIST = 0, 1, 2, 3, 4
 SCHED_LU_FORWARD = 5
+SCHED_SOLVE_ROWS = 6
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (

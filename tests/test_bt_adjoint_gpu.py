@@ -41,6 +41,7 @@ def test_adjoint_matches_dual_numbers_class_S(gpu, nsteps, ckpt, twist):
     zones, p = gpu.zones_for("BT", "S")
     ctx = gpu.Context("BT", zones, p.dt)
     if twist is not None:
+        ctx.set_schedule(gpu.native.SCHED_SOLVE_ROWS, 0)
         ctx.set_schedule(gpu.native.SCHED_SOLVE_TWIST, twist)
     ctx.init_exact()
     n = ctx.size

@@ -19,6 +19,8 @@ def _ctx(P, cls, strict, pair=None):
     zones, p = P.zones_for("BT", cls)
     ctx = P.Context("BT", zones, p.dt, strict=strict)
     if pair is not None:
+        # the one-wave and wave-pair kernels themselves, not the rows kernel
+        ctx.set_schedule(P.native.SCHED_SOLVE_ROWS, 0)
         ctx.set_schedule(P.native.SCHED_SOLVE_PAIR, pair)
     return ctx, zones, p
 
@@ -152,6 +154,7 @@ def test_two_sided_solve_tracks_oracle(gpu, cls):
     1e-12."""
     N = gpu.native
     ctx, zones, p = _ctx(gpu, cls, False)
+    ctx.set_schedule(N.SCHED_SOLVE_ROWS, 0)
     ctx.set_schedule(N.SCHED_SOLVE_TWIST, 7)
     o = OracleBT.for_class(p, threads=16)
     ctx.init_exact()

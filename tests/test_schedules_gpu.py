@@ -14,6 +14,8 @@ def _run(gpu, bench, cls, strict, grid, twist, nsteps=4):
     zones, p = gpu.zones_for(bench, cls)
     ctx = gpu.Context(bench, zones, p.dt, strict=strict)
     if twist is not None:
+        if bench == "BT" and not strict:
+            ctx.set_schedule(N.SCHED_SOLVE_ROWS, 0)  # the two-sided kernel itself
         ctx.set_schedule(N.SCHED_SOLVE_TWIST, twist)
     if grid is not None:
         ctx.set_schedule(N.SCHED_SOLVE_GRID, grid)
@@ -89,10 +91,14 @@ def test_schedule_change_discards_both_step_graphs(gpu, bench):
     res = []
     for mode in ("batched", "graph"):
         ctx = gpu.Context(bench, zones, p.dt)
-        ctx.set_schedule(N.SCHED_SOLVE_TWIST, 7)
+        # BT: the rows kernel, then the one-wave solves; SP: two-sided, then one-wave
+        knob = N.SCHED_SOLVE_ROWS if bench == "BT" else N.SCHED_SOLVE_TWIST
+        if bench == "BT":
+            ctx.set_schedule(N.SCHED_SOLVE_TWIST, 0)
+        ctx.set_schedule(knob, 7)
         ctx.init_exact()
         ctx.step(10, mode)
-        ctx.set_schedule(N.SCHED_SOLVE_TWIST, 0)
+        ctx.set_schedule(knob, 0)
         ctx.step(9, mode)
         res.append(ctx.get_all(0))
         ctx.close()
