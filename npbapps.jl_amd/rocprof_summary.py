@@ -13,12 +13,14 @@ a kernel change can never be priced with another kernel's traffic.
 Usage (after two --pmc runs, FETCH_SIZE then WRITE_SIZE, of the same command):
     python -m npbapps_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR OUT.json
     python -m npbapps_amd.rocprof_summary stats TRACE_DIR
+    python -m npbapps_amd.rocprof_summary f64 F64_DIR TRACE_DIR
 """
 import csv
 import glob
 import hashlib
 import json
 import os
+import re
 import subprocess
 import sys
 from collections import defaultdict
@@ -26,15 +28,17 @@ from collections import defaultdict
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 # the sources whose text determines the BT kernels' memory traffic
-KERNEL_SOURCES = ("csrc/npbmz.hip", "csrc/bt_kernels.inc", "csrc/npbmz_types.h")
+KERNEL_SOURCES = ("csrc/npbmz.hip", "csrc/bt_kernels.inc", "csrc/bt_rows.inc",
+                  "csrc/npbmz_types.h")
 COPY_KERNEL = "copy_probe_kernel"
-# phase name (bench.py) -> kernel name prefix of its default-schedule kernel
+# phase name (bench.py) -> pattern of the kernel that runs the phase (any of
+# the line-solve variants, by direction and whether add is folded in)
 PHASE_KERNELS = {
-    "exch": "npbmz::exch_kernel",
-    "rhs": "npbmz::fast::bt::rhs_kernel",
-    "x_solve": "<0, false>",
-    "y_solve": "<1, false>",
-    "z_solve+add": "<2, true>",
+    "exch": r"npbmz::exch_kernel<",
+    "rhs": r"npbmz::fast::bt::rhs_kernel<",
+    "x_solve": r"npbmz::fast::bt::solve\w*_kernel<0, false[,>]",
+    "y_solve": r"npbmz::fast::bt::solve\w*_kernel<1, false[,>]",
+    "z_solve+add": r"npbmz::fast::bt::solve\w*_kernel<2, true[,>]",
 }
 
 
@@ -97,15 +101,33 @@ def traffic(fetch_dir, write_dir, copy_bytes):
     wscale = copy_bytes / (wr[ck[0]] * 1024.0)
     rec = {"head": git_head(), "kernels_sha256": kernel_sources_sha256(),
            "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "kernels": {}}
-    for phase, needle in PHASE_KERNELS.items():
-        kf = [k for k in fe if needle in k and ("exch" in k or "bt::" in k)]
+    for phase, pat in PHASE_KERNELS.items():
+        kf = [k for k in fe if re.search(pat, k)]
         if not kf:
             continue
-        k = kf[0]
+        k = max(kf, key=lambda name: fe[name])
         b = fe[k] * 1024.0 * fscale + wr.get(k, 0.0) * 1024.0 * wscale
         rec[phase] = int(round(b))
         rec["kernels"][phase] = k
     return rec
+
+
+def f64_rates(f64_dir, trace_dir):
+    """(kernel, FMA, ADD, MUL, TRANS, GFLOP per launch, avg us, TFLOP/s)
+    with flops = 64 x (2 FMA + ADD + MUL + TRANS) per dispatch: every lane of
+    every issued FP64 wave instruction."""
+    names = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+             "SQ_INSTS_VALU_TRANS_F64")
+    c = [counter_per_dispatch(f64_dir, n) for n in names]
+    dur = {n: a for n, _, a, _, _, _ in kernel_stats(trace_dir)}
+    out = []
+    for k in c[0]:
+        if k not in dur:
+            continue
+        fma, add, mul, tr = (c[i].get(k, 0.0) for i in range(4))
+        gf = 64.0 * (2 * fma + add + mul + tr) / 1e9
+        out.append((k, fma, add, mul, tr, gf, dur[k], gf / dur[k] * 1e6 / 1e3))
+    return sorted(out, key=lambda x: -x[5])
 
 
 def main(argv):
@@ -121,6 +143,15 @@ def main(argv):
         print("| kernel | calls | avg us | min us | max us | % |\n|---|---|---|---|---|---|")
         for n, c, a, lo, hi, pc in kernel_stats(argv[1]):
             print(f"| `{n}` | {c} | {a:.1f} | {lo:.1f} | {hi:.1f} | {pc:.1f} |")
+        return 0
+    if len(argv) == 3 and argv[0] == "f64":
+        print("| kernel | FMA | ADD | MUL | TRANS | GFLOP per launch | avg us | TFLOP/s | of peak |")
+        print("|---|---|---|---|---|---|---|---|---|")
+        for k, fma, add, mul, tr, gf, us, tf in f64_rates(argv[1], argv[2]):
+            if gf < 0.01:
+                continue
+            print(f"| `{k.split('(')[0].replace('void ', '')}` | {fma:.3g} | {add:.3g} | {mul:.3g} | "
+                  f"{tr:.3g} | {gf:.3f} | {us:.1f} | {tf:.2f} | {tf / 78.6 * 100:.1f}% |")
         return 0
     print(__doc__)
     return 2
