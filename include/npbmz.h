@@ -184,9 +184,13 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
 
 /* Set one schedule knob directly (the tuner's candidates, for tests and
  * profiling).  No counterpart in the reference. */
-#define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone
-                                    (default: 1 for zones of at most 32768 interior
-                                    points, else 0) */
+#define NPBMZ_SCHED_LU_SWEEP 0   /* 0 hyperplane launches, 1 workgroup per zone,
+                                    2 each zone's hyperplanes split by k rows over
+                                    pipelined workgroups (tiles; FMA arithmetic,
+                                    where a tile's padded slots fit 512 threads;
+                                    otherwise hyperplane launches).  Default: 1
+                                    for zones of at most 32768 interior points,
+                                    else 2 (FMA) or 0 (strict) */
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup
                                     (default 512) */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else the size
@@ -225,6 +229,15 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     Default: x and y, and z while z's tasks
                                     of 12 lines all fit one round of two waves
                                     per SIMD */
+#define NPBMZ_SCHED_LU_TILES 7 /* LU pipelined sweep: tiles per zone, 1..8 (default 4;
+                                  fewer if a zone has fewer padded k rows) */
+#define NPBMZ_SCHED_LU_TILE_BATCH 8 /* LU pipelined sweep: hyperplanes a tile finishes
+                                       between publishing its progress, 1..64
+                                       (default 4) */
+#define NPBMZ_SCHED_LU_SPIN 9 /* LU pipelined sweep: polls a tile's wait for its
+                                 upstream tile may take before the step fails with
+                                 NPBMZ_EHIP (default 2^20; -1 fails every wait at
+                                 once, for tests of the error path) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -191,9 +191,11 @@ constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
 constexpr int kGraphGroup = 8; // timesteps per replay of the grouped graph
 constexpr int kMaxSolveGrid = 65536; // largest persistent solve grid accepted
 
-// LU sweep schedules: one launch per hyperplane over all zones, or one
-// workgroup per zone walking all hyperplanes of the sweep.
-enum { kSweepPlane = 0, kSweepZone = 1 };
+// LU sweep schedules: one launch per hyperplane over all zones, one
+// workgroup per zone walking all hyperplanes of the sweep, or a zone's
+// hyperplanes split over pipelined workgroups (tiles).
+enum { kSweepPlane = 0, kSweepZone = 1, kSweepTiles = 2 };
+constexpr int kMaxTiles = 8;  // tiles per zone of the pipelined sweep
 
 // A task table on the device plus, per zone, the range of its tasks.
 struct Table {
@@ -236,6 +238,12 @@ struct npbmz_ctx {
   int lu_fwd = 3;                 // per-zone sweep: forward each hyperplane's rsd (1) and u (2)
                                   // through LDS; 3: the padded-halo sweep where it applies
   size_t max_lds = 65536;         // dynamic LDS a zone-sweep workgroup may take
+  int lu_tiles = 4;               // pipelined sweep: workgroups (tiles) per zone
+  int lu_kb = 4;                  // pipelined sweep: hyperplanes per published count
+  int lu_spin = 1 << 20;          // pipelined sweep: polls before a wait gives up
+  double* txbuf = nullptr;        // pipelined sweep: tile edge rows of every hyperplane
+  int* tcnt = nullptr;            // pipelined sweep: finished hyperplanes per tile,
+                                  // then per zone the tiles done, then the error flag
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Table t_row[3];  // BT: tasks of fast::bt::kRowLines lines for solve_rows_kernel
   Face* faces = nullptr;
@@ -387,6 +395,28 @@ LuK make_lu_constants(double dt, double omega, int nx0, int ny0, int nz0) {
   }
   k.h5 = 2.0 * dt * k.c1345 * (t1[0] + t1[1] + t1[2]);
   return k;
+}
+
+// The zones by decreasing cost nx*ny*nz, exactly as map_zones orders them
+// before balancing them over processes (src/BT/setup_mpi.jl:123-157): a
+// selection sort that swaps the first strictly largest remaining zone to
+// the front.
+std::vector<int> cost_order(const npbmz_ctx& c) {
+  const int n = c.nz_;
+  std::vector<int> z(n);
+  std::iota(z.begin(), z.end(), 0);
+  auto size = [&](int q) { return double(c.nx[q]) * c.ny[q] * c.nz[q]; };
+  for (int i = 0; i + 1 < n; ++i) {
+    int m = i;
+    double cur = size(z[i]);
+    for (int j = i + 1; j < n; ++j)
+      if (cur < size(z[j])) {
+        cur = size(z[j]);
+        m = j;
+      }
+    if (m != i) std::swap(z[i], z[m]);
+  }
+  return z;
 }
 
 int lines_of(const npbmz_ctx* c, int z, int dir) {
@@ -730,6 +760,76 @@ bool lu_halo_ok(const npbmz_ctx* c) {
          (c->ny[0] - 1) * (c->nz[0] - 1) <= c->lu_nt && lu_halo_lds(c) <= c->max_lds;
 }
 
+// The pipelined tile sweep's shape: tiles per zone (none empty), slots per
+// tile, threads per workgroup (0 when a tile's slots exceed 512), hyperplane
+// steps and dynamic LDS.
+struct TileGeom {
+  int W, NS, nt, steps, SJ;
+  size_t lds;
+};
+TileGeom tile_geom(const npbmz_ctx* c) {
+  TileGeom g{};
+  const int NI = c->nx[0] - 2, NJ = c->ny[0] - 2, NK = c->nz[0] - 2;
+  const int NR = NK + 1;
+  g.SJ = NJ + 1;
+  int W = std::max(1, std::min(c->lu_tiles, NR));
+  const int rpt = (NR + W - 1) / W;
+  g.W = (NR + rpt - 1) / rpt;
+  g.NS = rpt * g.SJ;
+  g.nt = g.NS <= 128 ? 128 : g.NS <= 256 ? 256 : g.NS <= 512 ? 512 : 0;
+  g.steps = (NI + NJ + NK) & ~1;  // the halo hyperplane and ns interior ones, made even
+  g.lds = sizeof(double) * (2 * 10 * (size_t)g.NS + 1);  // the planes and one int
+  return g;
+}
+
+// The tile sweep applies with FMA arithmetic when a tile's slots fit a
+// workgroup and its LDS the device's.
+bool lu_tiles_ok(const npbmz_ctx* c) {
+  const TileGeom g = tile_geom(c);
+  return !(c->flags & NPBMZ_FLAG_STRICT_FP) && g.nt > 0 && g.lds <= c->max_lds;
+}
+
+// Exchange rows and counters of the tile sweep, allocated once per context
+// for up to kMaxTiles tiles per zone, zeroed (each launch leaves them zero).
+int ensure_tiles(npbmz_ctx* ctx) {
+  auto& c = *ctx;
+  if (c.txbuf) return NPBMZ_OK;
+  const TileGeom g = tile_geom(&c);
+  const size_t nx = (size_t)c.nz_ * kMaxTiles * g.steps * g.SJ * 10;
+  const size_t nc = (size_t)c.nz_ * kMaxTiles + c.nz_ + 1;
+  double* xb = nullptr;
+  int* cn = nullptr;
+  hipError_t e = hipMalloc(&xb, nx * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&cn, nc * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(cn, 0, nc * sizeof(int));
+  if (e != hipSuccess) {
+    if (xb) (void)hipFree(xb);
+    if (cn) (void)hipFree(cn);
+    return fail(ctx, e == hipErrorOutOfMemory ? NPBMZ_ENOMEM : NPBMZ_EHIP,
+                std::string("pipelined sweep buffers: ") + hipGetErrorString(e));
+  }
+  c.txbuf = xb;
+  c.tcnt = cn;
+  return NPBMZ_OK;
+}
+
+// After work that may have run tile sweeps: a wait that hit its bound shows
+// as the error flag, which is reported and cleared.
+int check_tiles(npbmz_ctx* ctx) {
+  auto& c = *ctx;
+  if (!c.tcnt) return NPBMZ_OK;
+  int* ef = c.tcnt + (size_t)c.nz_ * kMaxTiles + c.nz_;
+  int v = 0;
+  CK(hipMemcpyAsync(&v, ef, sizeof v, hipMemcpyDeviceToHost, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  if (v == 0) return NPBMZ_OK;
+  const int zero = 0;
+  CK(hipMemcpyAsync(ef, &zero, sizeof zero, hipMemcpyHostToDevice, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  return fail(ctx, NPBMZ_EHIP,
+              "LU pipelined sweep: a tile's wait for its upstream tile exceeded its bound");
+}
+
 // One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
 // zone (zone >= 0) or all zones, in the context's schedule.
 template <bool UPPER>
@@ -738,6 +838,26 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
   const int z0 = zone < 0 ? 0 : zone, nzl = zone < 0 ? c.nz_ : 1;
   const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
   const int ns = NI + NJ + NK - 2;
+  if (c.lu_sweep == kSweepTiles && lu_tiles_ok(&c)) {
+    // the exchange buffers come with the schedule (npbmz_set_schedule,
+    // npbmz_create): nothing is allocated here, where a graph may be capturing
+    if (!c.txbuf) return fail(ctx, NPBMZ_EHIP, "pipelined sweep buffers missing");
+    const TileGeom g = tile_geom(&c);
+    int* const cn = c.tcnt;
+    int* const dn = cn + (size_t)c.nz_ * kMaxTiles;
+    int* const ef = dn + c.nz_;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(nzl * g.W), dim3(g.nt), g.lds, c.stream, c.u, c.rhs, c.zd,
+                         z0, nzl, g.W, c.lu_kb, g.steps, c.txbuf, cn, dn, ef, c.lu_spin, c.lk);
+    };
+    switch (g.nt) {
+      case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128>); break;
+      case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256>); break;
+      default: go(fast::lu::sweep_tile_kernel<UPPER, 512>); break;
+    }
+    CK(hipGetLastError());
+    return NPBMZ_OK;
+  }
   if (c.lu_sweep == kSweepZone) {
     const int lvl = lu_fwd_level(&c);
     const size_t lds = lu_fwd_lds(&c, lvl);
@@ -1016,7 +1136,11 @@ int lu_step_store(npbmz_ctx* ctx, double* un, double* ue, double* r1, double* r2
 template <bool ASC>
 int k_lu_adj_sweep(npbmz_ctx* ctx, const double* ue, const double* ubar, double* nu, double* rho) {
   auto& c = *ctx;
-  if (c.lu_sweep == kSweepZone) {
+  // the pipelined tiles have no transposed form: their contexts run the
+  // transposed sweeps one workgroup per zone where a zone's pairs fit one
+  const bool tiles_as_zone = c.lu_sweep == kSweepTiles &&
+                             (c.ny[0] - 2) * (c.nz[0] - 2) <= c.lu_nt;
+  if (c.lu_sweep == kSweepZone || tiles_as_zone) {
     const int np = (c.ny[0] - 2) * (c.nz[0] - 2);
     if (c.lu_fwd == 3 && np <= c.lu_nt) {
       // the forward's level 3: hyperplanes handed on through LDS, own data
@@ -1197,7 +1321,12 @@ void bind_scratch(npbmz_ctx& c, int dev) {
 
 void describe(npbmz_ctx* c) {
   char b[256];
-  if (c->bench == NPBMZ_LU)
+  if (c->bench == NPBMZ_LU && c->lu_sweep == kSweepTiles && lu_tiles_ok(c)) {
+    const TileGeom g = tile_geom(c);
+    snprintf(b, sizeof b, "zones=%d sweep=pipelined-tiles tiles=%d batch=%d nt=%d exch=%s fp=%s",
+             c->nz_, g.W, c->lu_kb, g.nt, (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference",
+             strict(c) ? "strict" : "fma");
+  } else if (c->bench == NPBMZ_LU)
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
              c->lu_sweep == kSweepZone ? "zone-workgroup" : "hyperplane-launch", c->lu_nt,
              c->lu_sweep != kSweepZone ? ""
@@ -1313,6 +1442,11 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     const int64_t interior = int64_t(c.nx[0] - 2) * (c.ny[0] - 2) * (c.nz[0] - 2);
     if (interior > 32768) c.lu_sweep = kSweepPlane;
     else c.lu_nt = 512;
+    // zones too large for one workgroup's padded hyperplane run as pipelined
+    // tiles with FMA arithmetic (class B, 4 tiles publishing every 4
+    // hyperplanes: 0.905 ms per iteration against 1.095 with hyperplane
+    // launches; at class A the one-workgroup padded-halo sweep stays faster,
+    // 0.346 against 0.40 for the best split, 2 tiles)
     // hyperplane forwarding: the padded-halo sweep from 128 (j, k) pairs up
     // (class A, 420 pairs: 0.351 ms per iteration against 0.533 from global
     // memory); below, global memory (class W, 84 pairs: 0.127 against 0.131;
@@ -1326,12 +1460,9 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     base += (int64_t)c.nx[z] * c.ny[z] * c.nz[z];
   }
   c.npoints = base;
-  // largest zones first: the batched launches then schedule longest work first
-  c.order.resize(n);
-  std::iota(c.order.begin(), c.order.end(), 0);
-  std::stable_sort(c.order.begin(), c.order.end(), [&](int a, int b) {
-    return (int64_t)c.nx[a] * c.ny[a] * c.nz[a] > (int64_t)c.nx[b] * c.ny[b] * c.nz[b];
-  });
+  // largest zones first, in the reference's cost order: the batched
+  // launches then schedule the longest work first
+  c.order = cost_order(c);
   int rc = NPBMZ_OK;
   auto bad = [&](int code) {
     g_create_err = c.err;
@@ -1360,7 +1491,7 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
   CKC(hipMemset(c.rhs, 0, fb));
   if ((rc = build_tables(ctx))) return bad(rc);
   if (c.bench == NPBMZ_LU) {
-    // the zone sweeps' LDS: the device's per-workgroup maximum, granted to
+    // the zone and tile sweeps' LDS: the device's per-workgroup maximum, granted to
     // every instantiation that forwards hyperplanes (64 KiB if refused)
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1386,8 +1517,15 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       grant(fast::lu::sweep_zone_halo_kernel<false, 128>); grant(fast::lu::sweep_zone_halo_kernel<true, 128>);
       grant(fast::lu::sweep_zone_halo_kernel<false, 256>); grant(fast::lu::sweep_zone_halo_kernel<true, 256>);
       grant(fast::lu::sweep_zone_halo_kernel<false, 512>); grant(fast::lu::sweep_zone_halo_kernel<true, 512>);
+      grant(fast::lu::sweep_tile_kernel<false, 128>); grant(fast::lu::sweep_tile_kernel<true, 128>);
+      grant(fast::lu::sweep_tile_kernel<false, 256>); grant(fast::lu::sweep_tile_kernel<true, 256>);
+      grant(fast::lu::sweep_tile_kernel<false, 512>); grant(fast::lu::sweep_tile_kernel<true, 512>);
       (void)hipGetLastError();
       if (ok) c.max_lds = (size_t)v;
+    }
+    if (c.lu_sweep == kSweepPlane && !strict(ctx)) {
+      c.lu_sweep = kSweepTiles;
+      if (!lu_tiles_ok(ctx) || ensure_tiles(ctx) != NPBMZ_OK) c.lu_sweep = kSweepPlane;
     }
   }
   if (c.bench == NPBMZ_BT) {
@@ -1428,7 +1566,7 @@ void npbmz_destroy(npbmz_ctx* ctx) {
                  &c.t_face,   &c.t_face2,  &c.t_row[0],  &c.t_row[1],  &c.t_row[2]};
   for (Table* t : ts)
     if (t->d) (void)hipFree(t->d);
-  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart};
+  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart, c.txbuf, c.tcnt};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -1579,7 +1717,7 @@ int npbmz_step(npbmz_ctx* ctx, int32_t nsteps, int32_t mode, double* elapsed_ms)
     CK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     *elapsed_ms = ms;
   }
-  return NPBMZ_OK;
+  return check_tiles(ctx);
 }
 
 int npbmz_run_phase(npbmz_ctx* ctx, int32_t phase, int32_t mode) {
@@ -1587,7 +1725,7 @@ int npbmz_run_phase(npbmz_ctx* ctx, int32_t phase, int32_t mode) {
   int rc = run_phase(ctx, phase, mode);
   if (rc) return rc;
   CK(hipStreamSynchronize(ctx->stream));
-  return NPBMZ_OK;
+  return check_tiles(ctx);
 }
 
 int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
@@ -1635,6 +1773,7 @@ int npbmz_phase_times(npbmz_ctx* ctx, int32_t nsteps, double phase_ms[6]) {
   }
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (rc == NPBMZ_OK) rc = check_tiles(ctx);
   return rc;
 }
 
@@ -1810,8 +1949,9 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
   struct Knob { int which; std::vector<int> values; };
   std::vector<Knob> knobs;
   if (c.bench == NPBMZ_LU) {
-    knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}},
-             {NPBMZ_SCHED_LU_FORWARD, {3, 2, 1, 0}}};
+    knobs = {{NPBMZ_SCHED_LU_SWEEP, {1, 0, 2}}, {NPBMZ_SCHED_LU_THREADS, {128, 256, 512}},
+             {NPBMZ_SCHED_LU_FORWARD, {3, 2, 1, 0}}, {NPBMZ_SCHED_LU_TILES, {2, 4, 8}},
+             {NPBMZ_SCHED_LU_TILE_BATCH, {2, 4, 8}}};
   } else {
     knobs.push_back({-1, {1, 0}});
     if (c.bench == NPBMZ_BT) knobs.push_back({NPBMZ_SCHED_SOLVE_PAIR, {0, 2, 3, 7}});
@@ -1825,6 +1965,8 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
       case NPBMZ_SCHED_LU_SWEEP: return c.lu_sweep;
       case NPBMZ_SCHED_LU_THREADS: return c.lu_nt;
       case NPBMZ_SCHED_LU_FORWARD: return c.lu_fwd;
+      case NPBMZ_SCHED_LU_TILES: return c.lu_tiles;
+      case NPBMZ_SCHED_LU_TILE_BATCH: return c.lu_kb;
       case NPBMZ_SCHED_SOLVE_PAIR: return c.solve_pair;
       case NPBMZ_SCHED_SOLVE_TWIST: return c.solve_twist;
       case NPBMZ_SCHED_SOLVE_ROWS: return c.solve_rows;
@@ -1837,14 +1979,14 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
       if ((which == NPBMZ_SCHED_LU_THREADS || which == NPBMZ_SCHED_LU_FORWARD) &&
           c.lu_sweep != kSweepZone)
         return NPBMZ_OK;
+      if ((which == NPBMZ_SCHED_LU_TILES || which == NPBMZ_SCHED_LU_TILE_BATCH) &&
+          c.lu_sweep != kSweepTiles)
+        return NPBMZ_OK;
       return npbmz_set_schedule(ctx, which, v);
     }
     c.largest_first = v == 1;
-    std::iota(c.order.begin(), c.order.end(), 0);
-    if (c.largest_first)
-      std::stable_sort(c.order.begin(), c.order.end(), [&](int x, int y) {
-        return (int64_t)c.nx[x] * c.ny[x] * c.nz[x] > (int64_t)c.nx[y] * c.ny[y] * c.nz[y];
-      });
+    if (c.largest_first) c.order = cost_order(c);
+    else std::iota(c.order.begin(), c.order.end(), 0);
     drop_graphs(c);
     return build_tables(ctx);
   };
@@ -1898,9 +2040,29 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
   auto& c = *ctx;
   switch (which) {
     case NPBMZ_SCHED_LU_SWEEP:
-      if (c.bench != NPBMZ_LU || (value != 0 && value != 1))
-        return fail(ctx, NPBMZ_EINVAL, "LU sweep schedule is 0 (hyperplane) or 1 (zone)");
+      if (c.bench != NPBMZ_LU || value < 0 || value > 2)
+        return fail(ctx, NPBMZ_EINVAL,
+                    "LU sweep schedule is 0 (hyperplane), 1 (zone) or 2 (pipelined tiles)");
+      if (value == kSweepTiles) {
+        const int rc = ensure_tiles(ctx);
+        if (rc) return rc;
+      }
       c.lu_sweep = value;
+      break;
+    case NPBMZ_SCHED_LU_TILES:
+      if (c.bench != NPBMZ_LU || value < 1 || value > kMaxTiles)
+        return fail(ctx, NPBMZ_EINVAL, "LU tiles per zone is 1..8");
+      c.lu_tiles = value;
+      break;
+    case NPBMZ_SCHED_LU_TILE_BATCH:
+      if (c.bench != NPBMZ_LU || value < 1 || value > 64)
+        return fail(ctx, NPBMZ_EINVAL, "LU tile batch is 1..64 hyperplanes");
+      c.lu_kb = value;
+      break;
+    case NPBMZ_SCHED_LU_SPIN:
+      if (c.bench != NPBMZ_LU || value < -1)
+        return fail(ctx, NPBMZ_EINVAL, "LU tile wait bound is -1 or more polls");
+      c.lu_spin = value;
       break;
     case NPBMZ_SCHED_LU_THREADS:
       if (c.bench != NPBMZ_LU || (value != 128 && value != 256 && value != 512))
@@ -2069,6 +2231,7 @@ int npbmz_adjoint(npbmz_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const doub
   for (auto& x : e)
     if (x) (void)hipEventDestroy(x);
   release();
+  if (rc == NPBMZ_OK) rc = check_tiles(ctx);
   return rc;
 }
 

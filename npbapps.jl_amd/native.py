@@ -27,6 +27,7 @@ PHASE_SSOR_LOWER, PHASE_SSOR_UPPER, PHASE_SSOR_UPDATE = 2, 3, 4
 SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID, SCHED_SOLVE_PAIR, SCHED_SOLVE_TWIST = 0, 1, 2, 3, 4
 SCHED_LU_FORWARD = 5
 SCHED_SOLVE_ROWS = 6
+SCHED_LU_TILES, SCHED_LU_TILE_BATCH, SCHED_LU_SPIN = 7, 8, 9
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (
