@@ -226,9 +226,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     block rows on five lanes, two-sided, keeping
                                     the last reduced rows of each half in
                                     registers (overrides TWIST and PAIR).
-                                    Default: x and y, and z while z's tasks
-                                    of 12 lines all fit one round of two waves
-                                    per SIMD */
+                                    Default: all three directions (7) */
 #define NPBMZ_SCHED_LU_TILES 7 /* LU pipelined sweep: tiles per zone, 1..8 (default 4;
                                   fewer if a zone has fewer padded k rows) */
 #define NPBMZ_SCHED_LU_TILE_BATCH 8 /* LU pipelined sweep: hyperplanes a tile finishes

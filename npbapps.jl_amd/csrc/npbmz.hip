@@ -186,7 +186,7 @@ namespace {
 constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
-constexpr int kRowR = 8;     // reduced rows per half line kept in registers (rows kernel)
+constexpr int kRowR = 6;     // reduced rows per half line kept in registers (rows kernel)
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
 constexpr int kGraphGroup = 8; // timesteps per replay of the grouped graph
 constexpr int kMaxSolveGrid = 65536; // largest persistent solve grid accepted
@@ -1529,15 +1529,11 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     }
   }
   if (c.bench == NPBMZ_BT) {
-    // default, as measured on MI355X (ms/step, two-sided solves against the
-    // rows kernel in x and y, and in x, y and z): class S 0.057 / 0.043 /
-    // 0.042, W 0.082 / 0.065 / 0.063, A 0.173 / 0.148 / 0.153, B 0.396 /
-    // 0.328 / 0.332, C 1.421 / 1.318 / 1.331.  The rows kernel in z pays
-    // only while all of z's line tasks run in one round of two waves per SIMD
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    c.solve_rows = 3 | (c.t_row[2].n <= 4 * cus ? 4 : 0);
+    // default: the rows kernel in every direction, as measured on MI355X
+    // (ms/step, two-sided solves / rows kernel in x and y / in x, y and z):
+    // class S 0.057 / 0.039 / 0.037, W 0.083 / 0.056 / 0.054, A 0.172 /
+    // 0.120 / 0.120, B 0.397 / 0.304 / 0.296, C 1.421 / 1.243 / 1.236
+    c.solve_rows = 7;
   }
   if (c.bench == NPBMZ_SP) {
     // default: two-sided solves in the directions with fewer line tasks
