@@ -91,6 +91,8 @@ namespace exact {
 constexpr bool kIeeeDiv = true;
 // LU's jacld/jacu blocks with the reference's own operand grouping
 constexpr bool kCollapsedBlocks = false;
+// compute_rhs's dissipation loop by loop, as the reference adds it
+constexpr bool kBlendedDiss = false;
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 #include "bt_kernels.inc"
 #include "sp_kernels.inc"
@@ -108,6 +110,10 @@ constexpr bool kIeeeDiv = false;
 // LU's jacld/jacu blocks from per-point velocities and the collapsed
 // constant factors of LuK (about half the arithmetic)
 constexpr bool kCollapsedBlocks = true;
+// compute_rhs's dissipation as one five-point formula whose weights are the
+// reference's loops' coefficients summed for the point's line index: no
+// branch, where the loop form runs every loop's branch in most waves
+constexpr bool kBlendedDiss = true;
 __device__ __forceinline__ double recip(double x) {
   double r = __builtin_amdgcn_rcp(x);
   r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
