@@ -236,6 +236,17 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                  upstream tile may take before the step fails with
                                  NPBMZ_EHIP (default 2^20; -1 fails every wait at
                                  once, for tests of the error path) */
+#define NPBMZ_SCHED_LU_HANDOFF 10 /* LU pipelined sweep, how a tile hands its edge rows
+                                     on: 0 agent-scope release and acquire (an L2
+                                     write-back and invalidation per published count,
+                                     correct wherever the tiles run); 1 XCD-local: the
+                                     tiles of a zone share an XCD and its L2, and the
+                                     rows need no cache maintenance (the default where
+                                     the device places workgroup b on the XCD of
+                                     workgroup b mod 8, which is checked once per
+                                     device; EINVAL elsewhere); 2 as 1 with every tile's
+                                     placement check failing, for tests of the error
+                                     path (the step fails with NPBMZ_EHIP) */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

@@ -77,6 +77,12 @@ __global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
   for (int m = 0; m < 5; ++m) d[m] = s[m];
 }
 
+// Each workgroup's XCD: the XCC_ID hardware register (20, bits 0..3) of
+// block b into xcc[b].
+__global__ __launch_bounds__(64) void xcc_probe_kernel(int* __restrict__ xcc) {
+  if (threadIdx.x == 0) xcc[blockIdx.x] = (int)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11));
+}
+
 // Streaming copy used to calibrate byte counters (npbmz_probe_copy).
 __global__ __launch_bounds__(256) void copy_probe_kernel(const double4* __restrict__ src,
                                                          double4* __restrict__ dst, int64_t n) {
@@ -247,9 +253,14 @@ struct npbmz_ctx {
   int lu_tiles = 4;               // pipelined sweep: workgroups (tiles) per zone
   int lu_kb = 4;                  // pipelined sweep: hyperplanes per published count
   int lu_spin = 1 << 20;          // pipelined sweep: polls before a wait gives up
+  int lu_handoff = -1;            // pipelined sweep: 0 agent-scope, 1 XCD-local, 2 XCD-local
+                                  // with every placement check failing; -1 XCD-local where
+                                  // it applies
+  bool tile_local_ok = false;     // this device places blocks of equal b mod 8 on one XCD
   double* txbuf = nullptr;        // pipelined sweep: tile edge rows of every hyperplane
   int* tcnt = nullptr;            // pipelined sweep: finished hyperplanes per tile,
-                                  // then per zone the tiles done, then the error flag
+                                  // then per zone the tiles done, then the error flag,
+                                  // then each tile's XCD plus one (XCD-local hand-off)
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Table t_row[3];  // BT: tasks of fast::bt::kRowLines lines for solve_rows_kernel
   Face* faces = nullptr;
@@ -795,14 +806,43 @@ bool lu_tiles_ok(const npbmz_ctx* c) {
   return !(c->flags & NPBMZ_FLAG_STRICT_FP) && g.nt > 0 && g.lds <= c->max_lds;
 }
 
+// Whether the device places workgroups of equal b mod 8 on one XCD, as the
+// pipelined sweep's XCD-local hand-off needs: 1024 one-wave workgroups
+// report their XCC_ID register.  Measured once per device and process; the
+// sweep itself checks every pair of tiles it hands rows between.
+bool xcd_round_robin() {
+  static std::mutex mu;
+  static int state[64];        // 0 unknown, 1 round robin, 2 not
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+  std::lock_guard<std::mutex> lock(mu);
+  if (state[dev] == 0) {
+    constexpr int nb = 1024;
+    int* d = nullptr;
+    std::vector<int> h(nb, -1);
+    bool ok = hipMalloc(&d, nb * sizeof(int)) == hipSuccess;
+    if (ok) {
+      hipLaunchKernelGGL(xcc_probe_kernel, dim3(nb), dim3(64), 0, 0, d);
+      ok = hipGetLastError() == hipSuccess &&
+           hipMemcpy(h.data(), d, nb * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (d) (void)hipFree(d);
+    for (int b = 0; ok && b < nb; ++b) ok = h[b] >= 0 && h[b] < 16 && h[b] == h[b & 7];
+    state[dev] = ok ? 1 : 2;
+  }
+  return state[dev] == 1;
+}
+
 // Exchange rows and counters of the tile sweep, allocated once per context
-// for up to kMaxTiles tiles per zone, zeroed (each launch leaves them zero).
+// for up to kMaxTiles tiles per zone, zeroed (each launch leaves them zero),
+// and the device's workgroup placement for the XCD-local hand-off.
 int ensure_tiles(npbmz_ctx* ctx) {
   auto& c = *ctx;
   if (c.txbuf) return NPBMZ_OK;
+  c.tile_local_ok = xcd_round_robin();
   const TileGeom g = tile_geom(&c);
   const size_t nx = (size_t)c.nz_ * kMaxTiles * g.steps * g.SJ * 10;
-  const size_t nc = (size_t)c.nz_ * kMaxTiles + c.nz_ + 1;
+  const size_t nc = (size_t)c.nz_ * kMaxTiles * 2 + c.nz_ + 1;
   double* xb = nullptr;
   int* cn = nullptr;
   hipError_t e = hipMalloc(&xb, nx * sizeof(double));
@@ -833,7 +873,15 @@ int check_tiles(npbmz_ctx* ctx) {
   CK(hipMemcpyAsync(ef, &zero, sizeof zero, hipMemcpyHostToDevice, c.stream));
   CK(hipStreamSynchronize(c.stream));
   return fail(ctx, NPBMZ_EHIP,
-              "LU pipelined sweep: a tile's wait for its upstream tile exceeded its bound");
+              (v & 2) ? "LU pipelined sweep: a tile ran off its zone's XCD (XCD-local hand-off)"
+                      : "LU pipelined sweep: a tile's wait for its upstream tile exceeded its bound");
+}
+
+// The hand-off the pipelined sweep runs: XCD-local (1; 2 with a map that
+// no tile matches) or agent-scope (0).
+int tile_handoff(const npbmz_ctx* c) {
+  if (c->lu_handoff >= 0) return c->lu_handoff;
+  return c->tile_local_ok ? 1 : 0;
 }
 
 // One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
@@ -852,14 +900,28 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
     int* const cn = c.tcnt;
     int* const dn = cn + (size_t)c.nz_ * kMaxTiles;
     int* const ef = dn + c.nz_;
+    // tile w of zone zl is block w * nzp + zl: the same residue mod 8 for all
+    // tiles of a zone
+    const int nzp = (nzl + 7) & ~7;
+    const int ho = tile_handoff(&c);
+    int* const xc = ef + 1;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(nzl * g.W), dim3(g.nt), g.lds, c.stream, c.u, c.rhs, c.zd,
-                         z0, nzl, g.W, c.lu_kb, g.steps, c.txbuf, cn, dn, ef, c.lu_spin, c.lk);
+      hipLaunchKernelGGL(kern, dim3(nzp * g.W), dim3(g.nt), g.lds, c.stream, c.u, c.rhs, c.zd,
+                         z0, nzl, nzp, g.W, c.lu_kb, g.steps, c.txbuf, cn, dn, ef, c.lu_spin,
+                         xc, ho == 2 ? 1 : 0, c.lk);
     };
-    switch (g.nt) {
-      case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128>); break;
-      case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256>); break;
-      default: go(fast::lu::sweep_tile_kernel<UPPER, 512>); break;
+    if (ho != 0) {
+      switch (g.nt) {
+        case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128, true>); break;
+        case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256, true>); break;
+        default: go(fast::lu::sweep_tile_kernel<UPPER, 512, true>); break;
+      }
+    } else {
+      switch (g.nt) {
+        case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128, false>); break;
+        case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256, false>); break;
+        default: go(fast::lu::sweep_tile_kernel<UPPER, 512, false>); break;
+      }
     }
     CK(hipGetLastError());
     return NPBMZ_OK;
@@ -1329,8 +1391,9 @@ void describe(npbmz_ctx* c) {
   char b[256];
   if (c->bench == NPBMZ_LU && c->lu_sweep == kSweepTiles && lu_tiles_ok(c)) {
     const TileGeom g = tile_geom(c);
-    snprintf(b, sizeof b, "zones=%d sweep=pipelined-tiles tiles=%d batch=%d nt=%d exch=%s fp=%s",
-             c->nz_, g.W, c->lu_kb, g.nt, (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference",
+    snprintf(b, sizeof b, "zones=%d sweep=pipelined-tiles tiles=%d batch=%d nt=%d handoff=%s exch=%s fp=%s",
+             c->nz_, g.W, c->lu_kb, g.nt, tile_handoff(c) != 0 ? "xcd-local" : "agent",
+             (c->flags & NPBMZ_FLAG_LU_NPB_EXCH) ? "npb" : "reference",
              strict(c) ? "strict" : "fma");
   } else if (c->bench == NPBMZ_LU)
     snprintf(b, sizeof b, "zones=%d sweep=%s nt=%d%s exch=%s fp=%s", c->nz_,
@@ -1523,9 +1586,12 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       grant(fast::lu::sweep_zone_halo_kernel<false, 128>); grant(fast::lu::sweep_zone_halo_kernel<true, 128>);
       grant(fast::lu::sweep_zone_halo_kernel<false, 256>); grant(fast::lu::sweep_zone_halo_kernel<true, 256>);
       grant(fast::lu::sweep_zone_halo_kernel<false, 512>); grant(fast::lu::sweep_zone_halo_kernel<true, 512>);
-      grant(fast::lu::sweep_tile_kernel<false, 128>); grant(fast::lu::sweep_tile_kernel<true, 128>);
-      grant(fast::lu::sweep_tile_kernel<false, 256>); grant(fast::lu::sweep_tile_kernel<true, 256>);
-      grant(fast::lu::sweep_tile_kernel<false, 512>); grant(fast::lu::sweep_tile_kernel<true, 512>);
+      grant(fast::lu::sweep_tile_kernel<false, 128, false>); grant(fast::lu::sweep_tile_kernel<true, 128, false>);
+      grant(fast::lu::sweep_tile_kernel<false, 256, false>); grant(fast::lu::sweep_tile_kernel<true, 256, false>);
+      grant(fast::lu::sweep_tile_kernel<false, 512, false>); grant(fast::lu::sweep_tile_kernel<true, 512, false>);
+      grant(fast::lu::sweep_tile_kernel<false, 128, true>); grant(fast::lu::sweep_tile_kernel<true, 128, true>);
+      grant(fast::lu::sweep_tile_kernel<false, 256, true>); grant(fast::lu::sweep_tile_kernel<true, 256, true>);
+      grant(fast::lu::sweep_tile_kernel<false, 512, true>); grant(fast::lu::sweep_tile_kernel<true, 512, true>);
       (void)hipGetLastError();
       if (ok) c.max_lds = (size_t)v;
     }
@@ -2065,6 +2131,21 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench != NPBMZ_LU || value < -1)
         return fail(ctx, NPBMZ_EINVAL, "LU tile wait bound is -1 or more polls");
       c.lu_spin = value;
+      break;
+    case NPBMZ_SCHED_LU_HANDOFF:
+      if (c.bench != NPBMZ_LU || value < 0 || value > 2)
+        return fail(ctx, NPBMZ_EINVAL,
+                    "LU tile hand-off is 0 (agent scope), 1 (XCD-local) or 2 (XCD-local, "
+                    "placement check forced to fail)");
+      {
+        const int rc = ensure_tiles(ctx);
+        if (rc) return rc;
+      }
+      if (value == 1 && !c.tile_local_ok)
+        return fail(ctx, NPBMZ_EINVAL,
+                    "LU tile hand-off 1 needs workgroup b on the XCD of b mod 8, which this "
+                    "device does not do");
+      c.lu_handoff = value;
       break;
     case NPBMZ_SCHED_LU_THREADS:
       if (c.bench != NPBMZ_LU || (value != 128 && value != 256 && value != 512))

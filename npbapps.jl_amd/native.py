@@ -28,6 +28,7 @@ SCHED_LU_SWEEP, SCHED_LU_THREADS, SCHED_SOLVE_GRID, SCHED_SOLVE_PAIR, SCHED_SOLV
 SCHED_LU_FORWARD = 5
 SCHED_SOLVE_ROWS = 6
 SCHED_LU_TILES, SCHED_LU_TILE_BATCH, SCHED_LU_SPIN = 7, 8, 9
+SCHED_LU_HANDOFF = 10
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (

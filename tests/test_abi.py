@@ -21,6 +21,17 @@ def test_header_and_binding_list_the_same_entry_points():
     assert _declared() == set(native.EXPORTS)
 
 
+def test_binding_constants_match_the_header():
+    """Every NPBMZ_SCHED_*, NPBMZ_PHASE_* and NPBMZ_FLAG_* value of the header
+    is the binding's constant of the same name."""
+    hdr = open(os.path.join(ROOT, "include", "npbmz.h")).read()
+    defs = dict(re.findall(r"#define NPBMZ_((?:SCHED|PHASE|FLAG)_\w+)\s+\(?(\w+)", hdr))
+    assert defs
+    for name, val in defs.items():
+        assert hasattr(native, name), name
+        assert getattr(native, name) == int(val, 0), (name, val)
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(native.LIB_PATH)
     for name in _declared():

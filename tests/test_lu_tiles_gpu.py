@@ -1,5 +1,5 @@
 """LU's pipelined tile sweep (NPBMZ_SCHED_LU_SWEEP = 2) against the other
-sweep schedules and the CPU oracle.
+sweep schedules, its two hand-offs against each other, and the CPU oracle.
 
 Each zone's padded hyperplane is split by k rows over W workgroups that run
 the zone's hyperplanes as a pipeline, every tile handing its edge row to the
@@ -124,11 +124,50 @@ def test_tile_wait_bound_reports_an_error_and_recovers(gpu):
     ctx.close()
 
 
+@pytest.mark.parametrize("cls,tiles,batch", [("A", 4, 4), ("A", 2, 8), ("B", 4, 4), ("B", 8, 2)])
+def test_handoffs_agree_bitwise(gpu, cls, tiles, batch):
+    """The XCD-local hand-off (edge rows through the shared L2 of the zone's
+    XCD, no cache maintenance) and the agent-scope release/acquire give the
+    same bits: the rows they hand on are the same values."""
+    N = gpu.native
+    out = []
+    for ho in (0, 1):
+        ctx, zones, p = _ctx(gpu, cls, tiles=tiles, batch=batch)
+        ctx.set_schedule(N.SCHED_LU_HANDOFF, ho)
+        desc = ctx.describe()
+        assert ("handoff=xcd-local" if ho else "handoff=agent") in desc, desc
+        ctx.init_exact()
+        ctx.step(5, "graph")
+        out.append((ctx.get_all(0), ctx.get_all(2)))
+        ctx.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_handoff_placement_check_reports_an_error_and_recovers(gpu):
+    """Hand-off 2 runs the XCD-local kernel with a placement map that no
+    tile matches: every tile stops at its check, the step reports the
+    placement error, and with hand-off 1 the context runs a verified class W
+    run."""
+    N = gpu.native
+    ctx, zones, p = _ctx(gpu, "W")
+    ctx.set_schedule(N.SCHED_LU_HANDOFF, 2)
+    ctx.init_exact()
+    with pytest.raises(gpu.NpbmzError, match="XCD"):
+        ctx.step(2, "batched")
+    ctx.set_schedule(N.SCHED_LU_HANDOFF, 1)
+    ctx.init_exact()
+    ctx.step(p.niter, "graph")
+    xcr, xce, xci = ctx.norms()
+    assert gpu.verify.verify("LU", "W", p.dt, p.niter, xcr, xce, xci)[0]
+    ctx.close()
+
+
 def test_tile_knobs_are_validated(gpu):
     N = gpu.native
     ctx, zones, p = _ctx(gpu, "S")
     for which, bad in ((N.SCHED_LU_SWEEP, 3), (N.SCHED_LU_TILES, 0), (N.SCHED_LU_TILES, 9),
-                       (N.SCHED_LU_TILE_BATCH, 0), (N.SCHED_LU_SPIN, -2)):
+                       (N.SCHED_LU_TILE_BATCH, 0), (N.SCHED_LU_SPIN, -2),
+                       (N.SCHED_LU_HANDOFF, -1), (N.SCHED_LU_HANDOFF, 3)):
         with pytest.raises(gpu.NpbmzError):
             ctx.set_schedule(which, bad)
     ctx.close()
@@ -139,7 +178,8 @@ def test_default_sweeps_by_zone_size(gpu):
     hyperplane and run the padded-halo sweep; class B's do not and run the
     pipelined tiles; strict arithmetic (no tile kernel) keeps hyperplane
     launches there."""
-    for cls, strict, want in (("A", False, "lds-halo+prefetch"), ("B", False, "pipelined-tiles"),
+    for cls, strict, want in (("A", False, "lds-halo+prefetch"),
+                              ("B", False, "pipelined-tiles tiles=4 batch=4 nt=256 handoff=xcd-local"),
                               ("B", True, "hyperplane-launch")):
         zones, p = gpu.zones_for("LU", cls)
         ctx = gpu.Context("LU", zones, p.dt, omega=1.2, npb_exchange=True, strict=strict)
