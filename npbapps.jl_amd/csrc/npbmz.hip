@@ -199,6 +199,9 @@ constexpr int kPtNT = 128;   // threads per point task (rhs, add, init, norms)
 constexpr int kExchNT = 128; // threads per face task
 constexpr int kLineNT = 64;  // lanes (lines) per solve task: one wave
 constexpr int kRowR = 6;     // reduced rows per half line kept in registers (rows kernel)
+constexpr int kRowRLong = 8; // the same where a direction's tasks need more than one round of
+                             // waves: the rows kernel with its block rows read as applied
+                             // (fewer registers live), less spill traffic
 constexpr int kPlaneNT = 64; // threads per block of an LU hyperplane launch
 constexpr int kGraphGroup = 8; // timesteps per replay of the grouped graph
 constexpr int kMaxSolveGrid = 65536; // largest persistent solve grid accepted
@@ -242,6 +245,7 @@ struct npbmz_ctx {
   int solve_twist = 7;            // BT/SP, FMA only: bit d set: solve_twist_kernel (wins over pair)
   bool sp_twist_ok[3] = {false, false, false};  // SP: every line of direction d has >= 4 unknowns
   int solve_rows = 0;             // BT, FMA only: bit d set: solve_rows_kernel (wins over twist);
+  int cus = 256;                  // the device's compute units
                                   // set at creation by size
   int halfmax[3] = {0, 0, 0};     // rows of the longer half of the longest line per direction
   bool largest_first = true;      // task order: largest zones / longest lines first
@@ -654,6 +658,16 @@ int k_add(npbmz_ctx* ctx, int zone) {
 }
 
 // xraw (SP only) also receives each line's solution before its transform.
+// Whether the rows kernel runs its long-ring form in direction d: where the
+// direction's 12-line tasks (two waves each) exceed one round of two waves
+// per SIMD, it is bound by throughput and spill traffic rather than by one
+// wave's latency, and where its half lines are longer than the long ring
+// (shorter ones spill little either way, and the short form's reads are
+// issued together).
+bool rows_long(const npbmz_ctx& c, int d) {
+  return c.t_row[d].n > 4 * c.cus && c.halfmax[d] > kRowRLong;
+}
+
 template <int DIR, bool FUSE>
 int k_solve_t(npbmz_ctx* ctx, int zone, double* xraw = nullptr) {
   auto& c = *ctx;
@@ -681,8 +695,15 @@ int k_solve_t(npbmz_ctx* ctx, int zone, double* xraw = nullptr) {
     if (nr <= 0) return NPBMZ_OK;
     int gr = nr;
     if (zone < 0 && c.solve_grid[DIR] > 0) gr = std::min(nr, c.solve_grid[DIR]);
-    hipLaunchKernelGGL((fast::bt::solve_rows_kernel<DIR, FUSE, kRowR>), dim3(gr), dim3(128), 0,
-                       c.stream, c.u, c.rhs, c.cscr, c.zd, tr.d + f0, nr, c.halfmax[DIR], c.K);
+    // the same form for every launch mode (the forms round differently)
+    if (rows_long(c, DIR))
+      hipLaunchKernelGGL((fast::bt::solve_rows_kernel<DIR, FUSE, kRowRLong, true>), dim3(gr),
+                         dim3(128), 0, c.stream, c.u, c.rhs, c.cscr, c.zd, tr.d + f0, nr,
+                         c.halfmax[DIR], c.K);
+    else
+      hipLaunchKernelGGL((fast::bt::solve_rows_kernel<DIR, FUSE, kRowR, false>), dim3(gr),
+                         dim3(128), 0, c.stream, c.u, c.rhs, c.cscr, c.zd, tr.d + f0, nr,
+                         c.halfmax[DIR], c.K);
   } else if (!strict(ctx) && ((c.solve_twist >> DIR) & 1)) {
     go(fast::bt::solve_twist_kernel<DIR, FUSE>, 2 * kLineNT);
   } else if ((c.solve_pair >> DIR) & 1) {
@@ -1413,10 +1434,17 @@ void describe(npbmz_ctx* c) {
                                             (c->sp_twist_ok[2] ? 4 : 0))),
              strict(c) ? "strict" : "fma");
   else
-    snprintf(b, sizeof b, "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d rows=%d fp=%s",
+  {
+    // the rows kernel's ring of reduced rows per direction (0: not used)
+    int ring[3];
+    for (int d = 0; d < 3; ++d)
+      ring[d] = strict(c) || !((c->solve_rows >> d) & 1) ? 0 : rows_long(*c, d) ? kRowRLong : kRowR;
+    snprintf(b, sizeof b,
+             "zones=%d order=%s solve_grid=%d/%d/%d pair=%d twist=%d rows=%d ring=%d/%d/%d fp=%s",
              c->nz_, c->largest_first ? "largest-first" : "natural", c->solve_grid[0],
              c->solve_grid[1], c->solve_grid[2], c->solve_pair, strict(c) ? 0 : c->solve_twist,
-             strict(c) ? 0 : c->solve_rows, strict(c) ? "strict" : "fma");
+             strict(c) ? 0 : c->solve_rows, ring[0], ring[1], ring[2], strict(c) ? "strict" : "fma");
+  }
   c->sched = b;
 }
 
@@ -1483,6 +1511,7 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
 
   npbmz_ctx* ctx = new npbmz_ctx();
   auto& c = *ctx;
+  if (prop.multiProcessorCount > 0) c.cus = prop.multiProcessorCount;
   bind_scratch(c, dev);
   c.bench = p->bench; c.flags = flags; c.nz_ = n;
   c.x_zones = p->x_zones; c.y_zones = p->y_zones;

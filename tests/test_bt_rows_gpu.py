@@ -111,6 +111,19 @@ def test_rows_solve_persistent_grid_bitwise(gpu):
     assert np.array_equal(res[0], res[1])
 
 
+def test_ring_form_follows_the_task_count(gpu):
+    """Directions whose 12-line tasks exceed one round of two waves per SIMD
+    and whose half lines exceed 8 rows run the long-ring form (8 reduced rows
+    in registers, block rows read as applied), the others the short ring (6):
+    class A all short (x and y have few tasks, z short lines), class B long
+    in x and y on a 256-CU device."""
+    for cls, want in (("A", "ring=6/6/6"), ("B", "ring=8/8/6")):
+        zones, p = gpu.zones_for("BT", cls)
+        ctx = gpu.Context("BT", zones, p.dt)
+        assert want in ctx.describe(), (cls, ctx.describe())
+        ctx.close()
+
+
 def test_rows_solve_rejects_bad_masks_and_strict_ignores_it(gpu):
     N = gpu.native
     ctx, zones, p = _ctx(gpu, "S", rows=0)
