@@ -188,9 +188,11 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     2 each zone's hyperplanes split by k rows over
                                     pipelined workgroups (tiles; FMA arithmetic,
                                     where a tile's padded slots fit 512 threads;
-                                    otherwise hyperplane launches).  Default: 1
-                                    for zones of at most 32768 interior points,
-                                    else 2 (FMA) or 0 (strict) */
+                                    otherwise hyperplane launches).  Default
+                                    with FMA arithmetic: 2 where a zone's padded
+                                    hyperplane has more than 256 slots, else 1;
+                                    strict: 1 for zones of at most 32768
+                                    interior points, else 0 */
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup
                                     (default 512) */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else the size
@@ -227,8 +229,10 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     the last reduced rows of each half in
                                     registers (overrides TWIST and PAIR).
                                     Default: all three directions (7) */
-#define NPBMZ_SCHED_LU_TILES 7 /* LU pipelined sweep: tiles per zone, 1..8 (default 4;
-                                  fewer if a zone has fewer padded k rows) */
+#define NPBMZ_SCHED_LU_TILES 7 /* LU pipelined sweep: tiles per zone, 1..8 (default:
+                                  the padded slots over 256, rounded up, at most 8;
+                                  4 where the tiles are not the default; fewer if a
+                                  zone has fewer padded k rows) */
 #define NPBMZ_SCHED_LU_TILE_BATCH 8 /* LU pipelined sweep: hyperplanes a tile finishes
                                        between publishing its progress, 1..64
                                        (default 4) */

@@ -1540,11 +1540,12 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     const int64_t interior = int64_t(c.nx[0] - 2) * (c.ny[0] - 2) * (c.nz[0] - 2);
     if (interior > 32768) c.lu_sweep = kSweepPlane;
     else c.lu_nt = 512;
-    // zones too large for one workgroup's padded hyperplane run as pipelined
-    // tiles with FMA arithmetic (class B, 4 tiles publishing every 4
-    // hyperplanes: 0.905 ms per iteration against 1.095 with hyperplane
-    // launches; at class A the one-workgroup padded-halo sweep stays faster,
-    // 0.346 against 0.40 for the best split, 2 tiles)
+    // with FMA arithmetic, zones whose padded hyperplane exceeds 256 slots
+    // run as pipelined tiles of at most 256 slots each (below, npbmz_create:
+    // class A, 31x15 slots, 2 tiles: 0.300 ms per iteration against 0.327
+    // for the one-workgroup padded-halo sweep; class B, 51x16 slots, 4
+    // tiles: 0.686 against 1.095 with hyperplane launches; class W, 15x7
+    // slots, stays one workgroup per zone: 0.104 against 0.133 with 2 tiles)
     // hyperplane forwarding: the padded-halo sweep from 128 (j, k) pairs up
     // (class A, 420 pairs: 0.351 ms per iteration against 0.533 from global
     // memory); below, global memory (class W, 84 pairs: 0.127 against 0.131;
@@ -1624,9 +1625,15 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       (void)hipGetLastError();
       if (ok) c.max_lds = (size_t)v;
     }
-    if (c.lu_sweep == kSweepPlane && !strict(ctx)) {
+    const int slots = (c.ny[0] - 1) * (c.nz[0] - 1);
+    if (slots > 256 && !strict(ctx)) {
+      const int prev = c.lu_sweep;
       c.lu_sweep = kSweepTiles;
-      if (!lu_tiles_ok(ctx) || ensure_tiles(ctx) != NPBMZ_OK) c.lu_sweep = kSweepPlane;
+      c.lu_tiles = std::min(kMaxTiles, (slots + 255) / 256);
+      if (!lu_tiles_ok(ctx) || ensure_tiles(ctx) != NPBMZ_OK) {
+        c.lu_sweep = prev;
+        c.lu_tiles = 4;
+      }
     }
   }
   if (c.bench == NPBMZ_BT) {

@@ -152,13 +152,18 @@ def test_reference_exchange_run_matches_reference_as_written(gpu):
     assert abs(r["xci"] / oxci - 1) < 1e-10
 
 
-@pytest.mark.parametrize("npb", [False, True])
-def test_class_A_default_schedule_tracks_oracle_per_iteration(gpu, npb):
-    """Class A's default schedule (FMA arithmetic, the padded-halo zone
-    sweeps) with either exchange's planes: the norms within 1e-12 of the
-    oracle's after each of five iterations."""
+@pytest.mark.parametrize("npb,halo", [(False, False), (True, False), (False, True), (True, True)])
+def test_class_A_default_schedule_tracks_oracle_per_iteration(gpu, npb, halo):
+    """Class A's default schedule (FMA arithmetic, pipelined tiles of the
+    zones' padded hyperplanes) and the one-workgroup padded-halo zone sweep,
+    with either exchange's planes: the norms within 1e-12 of the oracle's
+    after each of five iterations."""
     ctx, zones, p = _ctx(gpu, "A", False, npb)
-    assert "lds-halo" in ctx.describe()
+    if halo:
+        ctx.set_schedule(gpu.native.SCHED_LU_SWEEP, 1)
+        assert "lds-halo" in ctx.describe()
+    else:
+        assert "pipelined-tiles tiles=2" in ctx.describe()
     o = OracleLU.for_class(p, flags=OracleLU.NPB_EXCH if npb else 0, threads=16)
     ctx.init_exact()
     for s in range(5):

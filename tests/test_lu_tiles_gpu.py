@@ -42,7 +42,8 @@ def _run(P, cls, n, mode="graph", **kw):
 
 @pytest.mark.parametrize("cls,ref_sweep", [("W", 1), ("A", 1), ("B", 0)])
 def test_tiles_match_other_sweeps(gpu, cls, ref_sweep):
-    """Several iterations with the tile sweep (4 tiles, batch 4) against the
+    """Several iterations with the tile sweep (the class's default split,
+    batch 4) against the
     workgroup-per-zone sweep (W: from global memory, A: the padded-halo form)
     and the hyperplane launches (B).  The point code and its neighbour values
     are the same; each kernel's FMA contraction is the compiler's, so they
@@ -59,7 +60,7 @@ def test_tiles_match_other_sweeps(gpu, cls, ref_sweep):
 @pytest.mark.parametrize("tiles,batch", [(2, 1), (3, 5), (8, 8), (4, 64)])
 def test_tile_counts_and_batches_agree_bitwise(gpu, tiles, batch):
     """Any split and publishing batch gives the same bits as the default
-    split (4 tiles, batch 4) at class A: only which workgroup computes a
+    split (2 tiles, batch 4) at class A: only which workgroup computes a
     point, and when its neighbours arrive, changes."""
     (u_ref, r_ref), _ = _run(gpu, "A", 4)
     (u, r), desc = _run(gpu, "A", 4, tiles=tiles, batch=batch)
@@ -174,11 +175,14 @@ def test_tile_knobs_are_validated(gpu):
 
 
 def test_default_sweeps_by_zone_size(gpu):
-    """The library's defaults: class A's zones fit one workgroup's padded
-    hyperplane and run the padded-halo sweep; class B's do not and run the
-    pipelined tiles; strict arithmetic (no tile kernel) keeps hyperplane
-    launches there."""
-    for cls, strict, want in (("A", False, "lds-halo+prefetch"),
+    """The library's defaults: zones whose padded hyperplane exceeds 256
+    slots run pipelined tiles of at most 256 slots (class A, 31x15 slots: 2
+    tiles; class B, 51x16: 4), smaller ones one workgroup per zone (class
+    W); strict arithmetic (no tile kernel) keeps the padded-halo-free zone
+    workgroups at class A and hyperplane launches at class B."""
+    for cls, strict, want in (("W", False, "zone-workgroup"),
+                              ("A", False, "pipelined-tiles tiles=2 batch=4 nt=256 handoff=xcd-local"),
+                              ("A", True, "zone-workgroup"),
                               ("B", False, "pipelined-tiles tiles=4 batch=4 nt=256 handoff=xcd-local"),
                               ("B", True, "hyperplane-launch")):
         zones, p = gpu.zones_for("LU", cls)
