@@ -907,8 +907,11 @@ int tile_handoff(const npbmz_ctx* c) {
 
 // One SSOR sweep (lower: rsd *= dt, jacld, blts; upper: jacu, buts) of one
 // zone (zone >= 0) or all zones, in the context's schedule.
+// upd (upper sweep, pipelined tiles or padded-halo zone sweep): the sweep
+// also takes ssor's update and stores u instead of rsd; *upd tells whether
+// it did (the other schedules leave the update to k_lu_update).
 template <bool UPPER>
-int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
+int k_lu_sweep_t(npbmz_ctx* ctx, int zone, bool* upd = nullptr) {
   auto& c = *ctx;
   const int z0 = zone < 0 ? 0 : zone, nzl = zone < 0 ? c.nz_ : 1;
   const int NI = c.nx[0] - 2, NJ = c.ny[0] - 2, NK = c.nz[0] - 2;
@@ -931,7 +934,22 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
                          z0, nzl, nzp, g.W, c.lu_kb, g.steps, c.txbuf, cn, dn, ef, c.lu_spin,
                          xc, ho == 2 ? 1 : 0, c.lk);
     };
-    if (ho != 0) {
+    if (UPPER && upd) {
+      *upd = true;
+      if (ho != 0) {
+        switch (g.nt) {
+          case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128, true, true>); break;
+          case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256, true, true>); break;
+          default: go(fast::lu::sweep_tile_kernel<UPPER, 512, true, true>); break;
+        }
+      } else {
+        switch (g.nt) {
+          case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128, false, true>); break;
+          case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256, false, true>); break;
+          default: go(fast::lu::sweep_tile_kernel<UPPER, 512, false, true>); break;
+        }
+      }
+    } else if (ho != 0) {
       switch (g.nt) {
         case 128: go(fast::lu::sweep_tile_kernel<UPPER, 128, true>); break;
         case 256: go(fast::lu::sweep_tile_kernel<UPPER, 256, true>); break;
@@ -964,10 +982,19 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
         hipLaunchKernelGGL(kern, dim3(nzl), dim3(nt), lu_halo_lds(&c), c.stream, c.u, c.rhs, c.zd,
                            z0, c.lk);
       };
-      switch (c.lu_nt) {
-        case 128: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 128>, 128); break;
-        case 512: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 512>, 512); break;
-        default: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 256>, 256); break;
+      if (UPPER && upd) {
+        *upd = true;
+        switch (c.lu_nt) {
+          case 128: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 128, true>, 128); break;
+          case 512: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 512, true>, 512); break;
+          default: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 256, true>, 256); break;
+        }
+      } else {
+        switch (c.lu_nt) {
+          case 128: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 128>, 128); break;
+          case 512: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 512>, 512); break;
+          default: goh(fast::lu::sweep_zone_halo_kernel<UPPER, 256>, 256); break;
+        }
       }
       CK(hipGetLastError());
       return NPBMZ_OK;
@@ -1005,8 +1032,8 @@ int k_lu_sweep_t(npbmz_ctx* ctx, int zone) {
   return NPBMZ_OK;
 }
 
-int k_lu_sweep(npbmz_ctx* ctx, bool upper, int zone) {
-  return upper ? k_lu_sweep_t<true>(ctx, zone) : k_lu_sweep_t<false>(ctx, zone);
+int k_lu_sweep(npbmz_ctx* ctx, bool upper, int zone, bool* upd = nullptr) {
+  return upper ? k_lu_sweep_t<true>(ctx, zone, upd) : k_lu_sweep_t<false>(ctx, zone);
 }
 
 // One LU iteration (src/LU/lu-node.jl:338-428): exch_qbc, then per zone
@@ -1015,18 +1042,21 @@ int k_lu_sweep(npbmz_ctx* ctx, bool upper, int zone) {
 int ssor_step(npbmz_ctx* ctx, int mode) {
   int rc;
   if ((rc = exch(ctx))) return rc;
+  // the upper sweep takes the update where its schedule can (upd)
   if (mode == NPBMZ_MODE_PERZONE) {
     for (int z = 0; z < ctx->nz_; ++z) {
+      bool upd = false;
       if ((rc = k_lu_sweep(ctx, false, z))) return rc;
-      if ((rc = k_lu_sweep(ctx, true, z))) return rc;
-      if ((rc = k_lu_update(ctx, z))) return rc;
+      if ((rc = k_lu_sweep(ctx, true, z, &upd))) return rc;
+      if (!upd && (rc = k_lu_update(ctx, z))) return rc;
       if ((rc = k_lu_rhs(ctx, z))) return rc;
     }
     return NPBMZ_OK;
   }
+  bool upd = false;
   if ((rc = k_lu_sweep(ctx, false, -1))) return rc;
-  if ((rc = k_lu_sweep(ctx, true, -1))) return rc;
-  if ((rc = k_lu_update(ctx, -1))) return rc;
+  if ((rc = k_lu_sweep(ctx, true, -1, &upd))) return rc;
+  if (!upd && (rc = k_lu_update(ctx, -1))) return rc;
   return k_lu_rhs(ctx, -1);
 }
 
@@ -1622,6 +1652,12 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
       grant(fast::lu::sweep_tile_kernel<false, 128, true>); grant(fast::lu::sweep_tile_kernel<true, 128, true>);
       grant(fast::lu::sweep_tile_kernel<false, 256, true>); grant(fast::lu::sweep_tile_kernel<true, 256, true>);
       grant(fast::lu::sweep_tile_kernel<false, 512, true>); grant(fast::lu::sweep_tile_kernel<true, 512, true>);
+      grant(fast::lu::sweep_zone_halo_kernel<true, 128, true>);
+      grant(fast::lu::sweep_zone_halo_kernel<true, 256, true>);
+      grant(fast::lu::sweep_zone_halo_kernel<true, 512, true>);
+      grant(fast::lu::sweep_tile_kernel<true, 128, false, true>); grant(fast::lu::sweep_tile_kernel<true, 128, true, true>);
+      grant(fast::lu::sweep_tile_kernel<true, 256, false, true>); grant(fast::lu::sweep_tile_kernel<true, 256, true, true>);
+      grant(fast::lu::sweep_tile_kernel<true, 512, false, true>); grant(fast::lu::sweep_tile_kernel<true, 512, true, true>);
       (void)hipGetLastError();
       if (ok) c.max_lds = (size_t)v;
     }

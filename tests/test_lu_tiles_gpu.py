@@ -189,3 +189,30 @@ def test_default_sweeps_by_zone_size(gpu):
         ctx = gpu.Context("LU", zones, p.dt, omega=1.2, npb_exchange=True, strict=strict)
         assert want in ctx.describe(), (cls, strict, ctx.describe())
         ctx.close()
+
+
+@pytest.mark.parametrize("cls,sweep", [("W", 1), ("A", 2), ("B", 2)])
+def test_update_in_upper_sweep_matches_separate_update(gpu, cls, sweep):
+    """A timestep's upper sweep takes ssor's update (u += rsd/(omega(2 -
+    omega))) itself in the padded-halo zone sweep (class W) and the
+    pipelined tiles (A, B); phase by phase, the upper sweep stores rsd and
+    the update runs as its own launch.  Both give the same bits in u.
+    (Class W runs the padded-halo form here; its default forwards nothing.)"""
+    N = gpu.native
+    out = []
+    for phases in (False, True):
+        ctx, zones, p = _ctx(gpu, cls, sweep=sweep)
+        if sweep == 1:
+            ctx.set_schedule(N.SCHED_LU_FORWARD, 3)  # the padded-halo form
+        assert ("pipelined-tiles" if sweep == 2 else "lds-halo") in ctx.describe()
+        ctx.init_exact()
+        for _ in range(3):
+            if phases:
+                for ph in (N.PHASE_EXCH, N.PHASE_SSOR_LOWER, N.PHASE_SSOR_UPPER,
+                           N.PHASE_SSOR_UPDATE, N.PHASE_RHS):
+                    ctx.run_phase(ph)
+            else:
+                ctx.step(1, "batched")
+        out.append(ctx.get_all(0))
+        ctx.close()
+    assert np.array_equal(out[0], out[1])
