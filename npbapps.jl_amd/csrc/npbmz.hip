@@ -1232,19 +1232,21 @@ int sp_adj_step(npbmz_ctx* ctx, const double* us, const double* r0, const double
 // ------------------------------------------------------------ LU adjoint
 // One batched LU iteration (exch_qbc, lower and upper sweeps, update, rhs),
 // copying u before and after the exchange and the outputs of the two sweeps
-// when the targets are given.
+// when the targets are given.  Without r2 the upper sweep may take the
+// update itself, as in ssor_step.
 int lu_step_store(npbmz_ctx* ctx, double* un, double* ue, double* r1, double* r2) {
   auto& c = *ctx;
   const size_t fb = sizeof(double) * 5 * (size_t)c.npoints;
   int rc;
+  bool upd = false;
   if (un) CK(hipMemcpyAsync(un, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
   if ((rc = exch(ctx))) return rc;
   if (ue) CK(hipMemcpyAsync(ue, c.u, fb, hipMemcpyDeviceToDevice, c.stream));
   if ((rc = k_lu_sweep(ctx, false, -1))) return rc;
   if (r1) CK(hipMemcpyAsync(r1, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
-  if ((rc = k_lu_sweep(ctx, true, -1))) return rc;
+  if ((rc = k_lu_sweep(ctx, true, -1, r2 ? nullptr : &upd))) return rc;
   if (r2) CK(hipMemcpyAsync(r2, c.rhs, fb, hipMemcpyDeviceToDevice, c.stream));
-  if ((rc = k_lu_update(ctx, -1))) return rc;
+  if (!upd && (rc = k_lu_update(ctx, -1))) return rc;
   return k_lu_rhs(ctx, -1);
 }
 
