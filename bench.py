@@ -332,7 +332,9 @@ def main():
         extra["sp_adjoint"] = sp_adjoint_case(P)
         extra["lu_adjoint"] = lu_adjoint_case(P)
         extra["sp_class_A"] = class_run(P, "SP", "A")
+        extra["sp_class_B"] = class_run(P, "SP", "B")
         extra["lu_class_A"] = class_run(P, "LU", "A", npb_exchange=True)
+        extra["lu_class_B"] = class_run(P, "LU", "B", npb_exchange=True)
 
     value = mflop_step * K / t_graph
     out = {
@@ -365,7 +367,9 @@ def main():
         "sp_adjoint": extra.get("sp_adjoint"),
         "lu_adjoint": extra.get("lu_adjoint"),
         "sp_class_A": extra.get("sp_class_A"),
+        "sp_class_B": extra.get("sp_class_B"),
         "lu_class_A": extra.get("lu_class_A"),
+        "lu_class_B": extra.get("lu_class_B"),
         "phase_ms_per_step": {n: round(per_launch_s[n] * 1e3, 5) for n in PHASES},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
