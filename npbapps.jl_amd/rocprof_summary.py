@@ -14,6 +14,7 @@ Usage (after two --pmc runs, FETCH_SIZE then WRITE_SIZE, of the same command):
     python -m npbapps_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR OUT.json
     python -m npbapps_amd.rocprof_summary stats TRACE_DIR
     python -m npbapps_amd.rocprof_summary f64 F64_DIR TRACE_DIR
+    python -m npbapps_amd.rocprof_summary sq PATTERN PMC_DIR [PMC_DIR ...]
 """
 import csv
 import glob
@@ -130,6 +131,37 @@ def f64_rates(f64_dir, trace_dir):
     return sorted(out, key=lambda x: -x[5])
 
 
+# SQ counters shown as shares of the waves' lifetime (SQ_WAVE_CYCLES)
+SQ_SHARES = (("SQ_ACTIVE_INST_VALU", "issuing VALU"), ("SQ_ACTIVE_INST_LDS", "issuing LDS"),
+             ("SQ_ACTIVE_INST_SCA", "issuing scalar"), ("SQ_WAIT_INST_ANY", "ready, not issued"),
+             ("SQ_WAIT_ANY", "waiting on counters or barrier"))
+
+
+def sq_table(pattern, dirs):
+    """Per kernel matching `pattern`: each SQ_SHARES counter as a share of
+    SQ_WAVE_CYCLES, and VALU and LDS instructions per wave, from --pmc runs
+    of the same command (one or more passes)."""
+    names = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_INSTS_VALU", "SQ_INSTS_LDS"] + [n for n, _ in SQ_SHARES]
+    val = defaultdict(dict)
+    for d in dirs:
+        for n in names:
+            try:
+                for k, v in counter_per_dispatch(d, n).items():
+                    if re.search(pattern, k):
+                        val[k][n] = v
+            except FileNotFoundError:
+                pass
+    rows = []
+    for k, v in val.items():
+        wc, wv = v.get("SQ_WAVE_CYCLES"), v.get("SQ_WAVES")
+        if not wc:
+            continue
+        shares = [100.0 * v[n] / wc if n in v else None for n, _ in SQ_SHARES]
+        per = [v[n] / wv if wv and n in v else None for n in ("SQ_INSTS_VALU", "SQ_INSTS_LDS")]
+        rows.append((k.split("(")[0].replace("void ", ""), wv, shares, per))
+    return rows
+
+
 def main(argv):
     if len(argv) >= 4 and argv[0] == "traffic":
         copy = int(argv[4]) if len(argv) > 4 else 512 << 20
@@ -152,6 +184,15 @@ def main(argv):
                 continue
             print(f"| `{k.split('(')[0].replace('void ', '')}` | {fma:.3g} | {add:.3g} | {mul:.3g} | "
                   f"{tr:.3g} | {gf:.3f} | {us:.1f} | {tf:.2f} | {tf / 78.6 * 100:.1f}% |")
+        return 0
+    if len(argv) >= 3 and argv[0] == "sq":
+        head = " | ".join(f"{label} %" for _, label in SQ_SHARES)
+        print(f"| kernel | waves | {head} | VALU insts per wave | LDS insts per wave |")
+        print("|---" * (4 + len(SQ_SHARES)) + "|")
+        f = lambda x, p: "-" if x is None else f"{x:.{p}f}"
+        for k, wv, shares, per in sq_table(argv[1], argv[2:]):
+            print(f"| `{k}` | {wv:.0f} | " + " | ".join(f(x, 1) for x in shares) +
+                  f" | {f(per[0], 0)} | {f(per[1], 0)} |")
         return 0
     print(__doc__)
     return 2
