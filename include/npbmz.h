@@ -194,7 +194,9 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     strict: 1 for zones of at most 32768
                                     interior points, else 0 */
 #define NPBMZ_SCHED_LU_THREADS 1 /* 128, 256 or 512 threads per zone workgroup
-                                    (default 512) */
+                                    (default with FMA arithmetic: the smallest that
+                                    holds a zone's hyperplane padded by a halo row
+                                    and column, up to 512; strict: 512) */
 #define NPBMZ_SCHED_SOLVE_GRID 2 /* BT/SP: 0 one block per task, else the size
                                     (1..65536) of a persistent grid striding
                                     over the tasks; NPBMZ_EINVAL otherwise */
@@ -220,8 +222,10 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                     column per thread, the zone's boundary values
                                     pass through LDS too and each thread's global
                                     loads are issued a hyperplane ahead;
-                                    otherwise as 2.  Default: 3 from 128 (j, k)
-                                    pairs per hyperplane up, else 0 */
+                                    otherwise as 2.  Default: with FMA arithmetic
+                                    3 where the padded hyperplane fits 512
+                                    threads; otherwise (and strict) 3 from 128
+                                    (j, k) pairs per hyperplane up, else 0 */
 #define NPBMZ_SCHED_SOLVE_ROWS 6 /* BT, FMA arithmetic only (ignored with
                                     NPBMZ_FLAG_STRICT_FP): bit d set: direction d
                                     solves 12 lines per wave with each line's five

@@ -1578,11 +1578,20 @@ int npbmz_create(const npbmz_problem* p, int32_t flags, npbmz_ctx** out) {
     // for the one-workgroup padded-halo sweep; class B, 51x16 slots, 4
     // tiles: 0.686 against 1.095 with hyperplane launches; class W, 15x7
     // slots, stays one workgroup per zone: 0.104 against 0.133 with 2 tiles)
-    // hyperplane forwarding: the padded-halo sweep from 128 (j, k) pairs up
-    // (class A, 420 pairs: 0.351 ms per iteration against 0.533 from global
-    // memory); below, global memory (class W, 84 pairs: 0.127 against 0.131;
-    // class S, 16 pairs: 0.051 against 0.060)
-    if ((c.ny[0] - 2) * (c.nz[0] - 2) < 128) c.lu_fwd = 0;
+    // hyperplane forwarding: with FMA arithmetic the padded-halo sweep in
+    // the smallest workgroup that holds a zone's padded hyperplane (class W,
+    // 15x7 slots, 128 threads: 0.088 ms per iteration against 0.105 with 512
+    // threads from global memory; class S: 0.040 against 0.044; class A, 465
+    // slots, 512 threads: 0.351 against 0.533 from global memory); strict
+    // arithmetic has no padded-halo form: rsd and u through LDS from 128
+    // (j, k) pairs up, else global memory
+    const int hslots = (c.ny[0] - 1) * (c.nz[0] - 1);
+    if (!strict(ctx) && hslots <= 512) {
+      c.lu_fwd = 3;
+      c.lu_nt = hslots <= 128 ? 128 : hslots <= 256 ? 256 : 512;
+    } else if ((c.ny[0] - 2) * (c.nz[0] - 2) < 128) {
+      c.lu_fwd = 0;
+    }
   }
   c.zh.resize(n);
   int64_t base = 0;
