@@ -255,6 +255,13 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms);
                                      device; EINVAL elsewhere); 2 as 1 with every tile's
                                      placement check failing, for tests of the error
                                      path (the step fails with NPBMZ_EHIP) */
+#define NPBMZ_SCHED_RHS_EXCH 11 /* BT and SP with more than one zone: 1 batched and
+                                   graph steps run the zone-face exchange in
+                                   compute_rhs's launch (the rhs reads each face
+                                   neighbour from the cell the exchange copies, the
+                                   exchange's blocks fill the halos for the line
+                                   solves; same bits); 0 a launch of its own;
+                                   default 1 */
 int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value);
 
 /* Describe the schedule currently in use (for logs); writes a NUL-terminated

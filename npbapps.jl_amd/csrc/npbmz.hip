@@ -47,14 +47,11 @@ struct Face {
 // faces reproduces it.  LU's faces overlap at the zone corners and its
 // sources include the corners of other faces, so LU runs two of these
 // launches (see build_tables).  Copies are bit exact.
-template <int NT>
-__global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
-                                                  const ZoneDev* __restrict__ zones,
-                                                  const Face* __restrict__ faces,
-                                                  const Task* __restrict__ tasks) {
-  const Task T = tasks[blockIdx.x];
+// Copy t of face task T.
+__device__ __forceinline__ void exch_copy(double* u, const ZoneDev* __restrict__ zones,
+                                          const Face* __restrict__ faces, const Task T, int t) {
   const Face F = faces[T.zone];
-  const int idx = T.first + (int)threadIdx.x;
+  const int idx = T.first + t;
   if (idx >= F.npts) return;
   const ZoneDev D = zones[F.dst], S = zones[F.src];
   int64_t dp, sp;
@@ -71,11 +68,28 @@ __global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
     dp = D.base + ((int64_t)k * D.ny + dj) * D.nx + i;
     sp = S.base + ((int64_t)k * S.ny + sj) * S.nx + i;
   }
-  const double* __restrict__ s = u + sp * 5;
-  double* __restrict__ d = u + dp * 5;
+  const double* s = u + sp * 5;
+  double* d = u + dp * 5;
 #pragma unroll
   for (int m = 0; m < 5; ++m) d[m] = s[m];
 }
+
+template <int NT>
+__global__ __launch_bounds__(NT) void exch_kernel(double* __restrict__ u,
+                                                  const ZoneDev* __restrict__ zones,
+                                                  const Face* __restrict__ faces,
+                                                  const Task* __restrict__ tasks) {
+  exch_copy(u, zones, faces, tasks[blockIdx.x], (int)threadIdx.x);
+}
+
+// Where a BT/SP zone's halo cells come from in the exchange, per zone and
+// side (0 i = 0, 1 i = nx-1, 2 j = 0, 3 j = ny-1): the halo cell at
+// transverse index t (j for x sides, i for y sides) and plane k holds the
+// source zone's point base + t * st + k * sk (exch_copy's source).  A side
+// that receives nothing has st = sk = 0 and base = -1.
+struct HaloSrc {
+  int64_t base, st, sk;
+};
 
 // Each workgroup's XCD: the XCC_ID hardware register (20, bits 0..3) of
 // block b into xcc[b].
@@ -267,6 +281,9 @@ struct npbmz_ctx {
                                   // then each tile's XCD plus one (XCD-local hand-off)
   Table t_int, t_all, t_line[3], t_face, t_face2;
   Table t_row[3];  // BT: tasks of fast::bt::kRowLines lines for solve_rows_kernel
+  HaloSrc* hsrc = nullptr;  // BT/SP: where each zone side's halo cells come from
+  bool hsrc_ok = false;     // hsrc describes the face table exactly
+  int rhs_exch = 1;         // BT/SP batched steps: 1 exchange inside compute_rhs's launch
   Face* faces = nullptr;
   double* partial = nullptr;
   size_t partial_n = 0;     // blocks the partial sums are sized for
@@ -584,6 +601,32 @@ int build_tables(npbmz_ctx* ctx) {
   }
   if (!c.faces) CK(hipMalloc(&c.faces, sizeof(Face) * fh.size()));
   CK(hipMemcpy(c.faces, fh.data(), sizeof(Face) * fh.size(), hipMemcpyHostToDevice));
+  // BT/SP: the face table by destination side, for compute_rhs with the
+  // exchange in its launch (exch_copy's source of each halo cell)
+  if (c.bench != NPBMZ_LU && !c.hsrc) {
+    std::vector<HaloSrc> hs((size_t)4 * c.nz_, HaloSrc{-1, 0, 0});
+    bool ok = true;
+    for (const Face& f : fh) {
+      const int D = f.dst, S = f.src;
+      const ZoneDev& zs = c.zh[S];
+      const bool xf = f.type < 2;
+      const int tn = xf ? c.ny[D] - 2 : c.nx[D] - 2;
+      ok = ok && hs[4 * D + f.type].base < 0 && f.t0 == 1 && f.tn == tn && f.k0 == 1 &&
+           f.npts == tn * (c.nz[D] - 2) && c.nz[S] == c.nz[D] &&
+           (xf ? c.ny[S] == c.ny[D] : c.nx[S] == c.nx[D]);
+      if (!ok) break;
+      const int64_t sk = (int64_t)zs.nx * zs.ny;
+      hs[4 * D + f.type] = f.type == 0   ? HaloSrc{zs.base + zs.nx - 2, zs.nx, sk}
+                           : f.type == 1 ? HaloSrc{zs.base + 1, zs.nx, sk}
+                           : f.type == 2 ? HaloSrc{zs.base + (int64_t)(zs.ny - 2) * zs.nx, 1, sk}
+                                         : HaloSrc{zs.base + zs.nx, 1, sk};
+    }
+    HaloSrc* d = nullptr;
+    CK(hipMalloc(&d, sizeof(HaloSrc) * hs.size()));
+    CK(hipMemcpy(d, hs.data(), sizeof(HaloSrc) * hs.size(), hipMemcpyHostToDevice));
+    c.hsrc = d;
+    c.hsrc_ok = ok;
+  }
   if ((rc = build_table(ctx, c.t_face, fa, kExchNT, [&](int f) { return fh[f].npts; })))
     return rc;
   if ((rc = build_table(ctx, c.t_face2, fb, kExchNT, [&](int f) { return fh[f].npts; })))
@@ -621,13 +664,32 @@ bool strict(const npbmz_ctx* c) { return (c->flags & NPBMZ_FLAG_STRICT_FP) != 0;
 // compute_rhs.  For SP the timestep's rhs phase is compute_rhs + txinvr;
 // plain = true gives compute_rhs alone (SP's verify, src/SP/verify.jl:436).
 // u, frc, rhs default to the context's fields.
+// With_exch (all zones, the context's fields, rhs_exch_fusable): the step's
+// zone-face exchange runs in the same launch.
 int k_rhs(npbmz_ctx* ctx, int zone, bool plain = false, double* r0out = nullptr,
-          const double* u = nullptr, const double* frc = nullptr, double* rhs = nullptr) {
+          const double* u = nullptr, const double* frc = nullptr, double* rhs = nullptr,
+          bool with_exch = false) {
   auto& c = *ctx;
   const Table& t = c.t_int;
   int first = zone < 0 ? 0 : t.zfirst[zone], n = zone < 0 ? t.n : t.zcount[zone];
   if (n <= 0) return NPBMZ_OK;
   const bool tx = c.bench == NPBMZ_SP && !plain;
+  if (with_exch) {
+    auto gx = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(t.n + c.t_face.n), dim3(kPtNT), 0, c.stream, c.u, c.frc,
+                         c.rhs, c.zd, t.d, t.n, c.K, tx ? r0out : nullptr, c.hsrc, c.faces,
+                         c.t_face.d);
+    };
+    if (strict(ctx)) {
+      if (tx) gx(exact::bt::rhs_exch_kernel<kPtNT, true>);
+      else gx(exact::bt::rhs_exch_kernel<kPtNT, false>);
+    } else {
+      if (tx) gx(fast::bt::rhs_exch_kernel<kPtNT, true>);
+      else gx(fast::bt::rhs_exch_kernel<kPtNT, false>);
+    }
+    CK(hipGetLastError());
+    return NPBMZ_OK;
+  }
   if (!u) u = c.u;
   if (!frc) frc = c.frc;
   if (!rhs) rhs = c.rhs;
@@ -1066,8 +1128,25 @@ int ssor_step(npbmz_ctx* ctx, int mode) {
 // per-zone order (all zones' rhs, then all x solves, ...) is equivalent to
 // the reference's zone-by-zone adi because zones are independent after the
 // exchange.
+// Whether a batched BT/SP step runs its exchange inside compute_rhs's
+// launch: more than one zone (so there is an exchange), one exchange launch
+// (kExchNT-point face tasks, as exch_copy takes) and a face table hsrc
+// describes.
+bool rhs_exch_fusable(const npbmz_ctx* c) {
+  return c->bench != NPBMZ_LU && c->rhs_exch == 1 && c->nz_ > 1 && c->hsrc_ok &&
+         c->t_face2.n == 0 && c->t_face.n > 0 && kExchNT == kPtNT;
+}
+
 int adi_step(npbmz_ctx* ctx, int mode) {
   int rc;
+  if (mode != NPBMZ_MODE_PERZONE && rhs_exch_fusable(ctx)) {
+    // per-zone mode keeps them apart: a zone's add would otherwise reach
+    // another zone's rhs in the same step
+    if ((rc = k_rhs(ctx, -1, false, nullptr, nullptr, nullptr, nullptr, true))) return rc;
+    if ((rc = k_solve(ctx, 0, false, -1))) return rc;
+    if ((rc = k_solve(ctx, 1, false, -1))) return rc;
+    return k_solve(ctx, 2, true, -1);
+  }
   if ((rc = exch(ctx))) return rc;
   if (mode == NPBMZ_MODE_PERZONE) {
     for (int z = 0; z < ctx->nz_; ++z) {
@@ -1717,7 +1796,8 @@ void npbmz_destroy(npbmz_ctx* ctx) {
                  &c.t_face,   &c.t_face2,  &c.t_row[0],  &c.t_row[1],  &c.t_row[2]};
   for (Table* t : ts)
     if (t->d) (void)hipFree(t->d);
-  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart, c.txbuf, c.tcnt};
+  void* ps[] = {c.zd, c.u, c.frc, c.rhs, c.cscr, c.faces, c.partial, c.zpart, c.txbuf, c.tcnt,
+                c.hsrc};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -2110,6 +2190,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
     if (c.bench == NPBMZ_BT && !strict(ctx))
       knobs.push_back({NPBMZ_SCHED_SOLVE_ROWS, {0, 1, 3, 7}});
     knobs.push_back({NPBMZ_SCHED_SOLVE_GRID, {0, 1024}});
+    knobs.push_back({NPBMZ_SCHED_RHS_EXCH, {1, 0}});
   }
   auto get = [&](int which) -> int {
     switch (which) {
@@ -2122,6 +2203,7 @@ int npbmz_tune(npbmz_ctx* ctx, int32_t budget_ms) {
       case NPBMZ_SCHED_SOLVE_TWIST: return c.solve_twist;
       case NPBMZ_SCHED_SOLVE_ROWS: return c.solve_rows;
       case NPBMZ_SCHED_SOLVE_GRID: return c.solve_grid[0];
+      case NPBMZ_SCHED_RHS_EXCH: return c.rhs_exch;
       default: return c.largest_first ? 1 : 0;
     }
   };
@@ -2254,6 +2336,11 @@ int npbmz_set_schedule(npbmz_ctx* ctx, int32_t which, int32_t value) {
       if (c.bench != NPBMZ_BT || value < 0 || value > 7)
         return fail(ctx, NPBMZ_EINVAL, "BT rows-over-lanes solve mask is 0..7");
       c.solve_rows = value;
+      break;
+    case NPBMZ_SCHED_RHS_EXCH:
+      if (c.bench == NPBMZ_LU || value < 0 || value > 1)
+        return fail(ctx, NPBMZ_EINVAL, "BT/SP exchange inside compute_rhs is 0 or 1");
+      c.rhs_exch = value;
       break;
     case NPBMZ_SCHED_SOLVE_GRID:
       // a launch takes min(tasks, grid) workgroups, so the line scratch

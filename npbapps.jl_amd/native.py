@@ -29,6 +29,7 @@ SCHED_LU_FORWARD = 5
 SCHED_SOLVE_ROWS = 6
 SCHED_LU_TILES, SCHED_LU_TILE_BATCH, SCHED_LU_SPIN = 7, 8, 9
 SCHED_LU_HANDOFF = 10
+SCHED_RHS_EXCH = 11
 
 # Every symbol include/npbmz.h declares.
 EXPORTS = (

@@ -142,3 +142,40 @@ def test_rows_solve_rejects_bad_masks_and_strict_ignores_it(gpu):
         assert np.array_equal(ctx.get_dense(z, 0), o.get(z, OracleBT.U))
     ctx.close()
     o.close()
+
+
+@pytest.mark.parametrize("bench,cls", [("BT", "W"), ("BT", "A"), ("SP", "W"), ("SP", "A")])
+def test_exchange_in_rhs_launch_matches_separate_exchange(gpu, bench, cls):
+    """Batched and graph steps run the zone-face exchange in compute_rhs's
+    launch by default (the rhs reads each face neighbour from the cell the
+    exchange copies; the exchange's blocks fill the halos for the solves);
+    NPBMZ_SCHED_RHS_EXCH = 0 gives the exchange a launch of its own.  Both
+    give the same bits in u, halos included, after a group of eight graph
+    steps and a single one."""
+    N = gpu.native
+    zones, p = gpu.zones_for(bench, cls)
+    out = []
+    for fx in (1, 0):
+        ctx = gpu.Context(bench, zones, p.dt)
+        ctx.set_schedule(N.SCHED_RHS_EXCH, fx)
+        ctx.init_exact()
+        ctx.step(9, "graph")
+        ctx.step(2, "batched")
+        out.append(ctx.get_all(0))
+        ctx.close()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_rhs_exch_knob_is_validated(gpu):
+    N = gpu.native
+    zones, p = gpu.zones_for("BT", "S")
+    ctx = gpu.Context("BT", zones, p.dt)
+    for bad in (-1, 2):
+        with pytest.raises(gpu.NpbmzError):
+            ctx.set_schedule(N.SCHED_RHS_EXCH, bad)
+    ctx.close()
+    zones, p = gpu.zones_for("LU", "S")
+    ctx = gpu.Context("LU", zones, p.dt, omega=1.2)
+    with pytest.raises(gpu.NpbmzError):
+        ctx.set_schedule(N.SCHED_RHS_EXCH, 1)
+    ctx.close()
